@@ -1,0 +1,113 @@
+"""ctypes binding of libh4hip.so (include/h4hip.h).
+
+The library is loaded from the package directory only; if it is missing
+the import fails loudly -- there is no CPU fallback for any entry point.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libh4hip.so")
+
+H4H_OK = 0
+H4H_EINVAL = -1
+H4H_EHIP = -2
+H4H_ENOMEM = -3
+H4H_ESTATE = -4
+H4H_ENODEV = -5
+
+H4H_FP64 = 0
+H4H_FP64_EXACT = 1
+H4H_MIXED = 2
+
+PRECISIONS = {"fp64": H4H_FP64, "exact": H4H_FP64_EXACT, "mixed": H4H_MIXED}
+
+# Every symbol include/h4hip.h declares (checked by the CPU test suite).
+EXPORTS = (
+    "h4h_abi_version", "h4h_last_error", "h4h_create", "h4h_destroy",
+    "h4h_set_precision", "h4h_set_launch", "h4h_upload", "h4h_download",
+    "h4h_download_predictor", "h4h_upload_predictor", "h4h_init_acc_jrk",
+    "h4h_init_dt", "h4h_find_particles_to_move", "h4h_set_active",
+    "h4h_get_active", "h4h_save_old_acc_jrk", "h4h_predicted_pos_vel",
+    "h4h_update_acc_jrk", "h4h_correction_pos_vel",
+    "h4h_next_integration_time", "h4h_step", "h4h_get_energy",
+    "h4h_set_timing", "h4h_get_stats", "h4h_synchronize",
+)
+
+
+class H4HStats(ctypes.Structure):
+    _fields_ = [
+        ("block_steps", ctypes.c_uint64),
+        ("interactions", ctypes.c_uint64),
+        ("nsteps", ctypes.c_uint64),
+        ("update_launches", ctypes.c_uint64),
+        ("update_ms", ctypes.c_double),
+        ("step_ms", ctypes.c_double),
+    ]
+
+
+class H4HError(RuntimeError):
+    def __init__(self, func, code, msg):
+        super().__init__(f"{func} failed ({code}): {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Loads libh4hip.so and declares its prototypes."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    u32, i32, dbl, flt = ctypes.c_uint32, ctypes.c_int, ctypes.c_double, ctypes.c_float
+    pd = ctypes.POINTER(ctypes.c_double)
+    pu = ctypes.POINTER(ctypes.c_uint32)
+    pf = ctypes.POINTER(ctypes.c_float)
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "h4h_abi_version": ([], i32),
+        "h4h_last_error": ([], ctypes.c_char_p),
+        "h4h_create": ([pp, u32, dbl, flt, dbl, dbl, i32], i32),
+        "h4h_destroy": ([P], i32),
+        "h4h_set_precision": ([P, i32], i32),
+        "h4h_set_launch": ([P, u32, u32], i32),
+        "h4h_upload": ([P, pd, pd, pd, pd, pd], i32),
+        "h4h_download": ([P, pd, pd, pd, pd, pd, pd, pd, pd], i32),
+        "h4h_download_predictor": ([P, pd, pf], i32),
+        "h4h_upload_predictor": ([P, pd, pf], i32),
+        "h4h_init_acc_jrk": ([P], i32),
+        "h4h_init_dt": ([P, pd, flt, dbl], i32),
+        "h4h_find_particles_to_move": ([P, dbl, pu], i32),
+        "h4h_set_active": ([P, pu, u32], i32),
+        "h4h_get_active": ([P, pu, pu], i32),
+        "h4h_save_old_acc_jrk": ([P], i32),
+        "h4h_predicted_pos_vel": ([P, dbl], i32),
+        "h4h_update_acc_jrk": ([P, u32], i32),
+        "h4h_correction_pos_vel": ([P, dbl, u32], i32),
+        "h4h_next_integration_time": ([P, pd], i32),
+        "h4h_step": ([P, dbl, pu, pd], i32),
+        "h4h_get_energy": ([P, pd, pd], i32),
+        "h4h_set_timing": ([P, i32], i32),
+        "h4h_get_stats": ([P, ctypes.POINTER(H4HStats), i32], i32),
+        "h4h_synchronize": ([P], i32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def check(func, rc):
+    if rc != H4H_OK:
+        msg = load().h4h_last_error().decode(errors="replace")
+        raise H4HError(func, rc, msg)
+    return rc

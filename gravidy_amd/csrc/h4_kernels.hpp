@@ -1,0 +1,159 @@
+// h4_kernels.hpp -- gfx950 device code of the Hermite-4 block-timestep path.
+//
+// State lives in HBM as structure-of-arrays (one double array per
+// component) so every kernel's per-particle loads are coalesced 8-byte
+// lanes.  The pairwise kernels stage j particles through LDS in tiles of
+// JT and read them back as wave-uniform (broadcast) LDS loads.
+//
+// Rounding rules.  Predictor, corrector, timestep, init_dt and energy are
+// compiled with FP contraction off and use IEEE division and sqrt, so each
+// operation rounds exactly as the reference's x86-64 build does
+// (Hermite4CPU.cpp:340-411, NbodyUtils.cpp:337-529, Hermite4.cpp:108-140).
+// The FP64_EXACT pair kernel does the same for the pair terms and keeps the
+// ascending-j order of Hermite4CPU::update_acc_jrk.  The FP64 pair kernel
+// contracts to FMA, uses a refined v_rsq_f64 and may split the j range.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace h4 {
+
+constexpr int BS = 256;           // threads per workgroup: 4 wave64
+constexpr int JT = 256;           // j particles per LDS tile (== BS)
+constexpr double PAD_POS = 1.0e18;  // position of zero-mass padding particles
+
+// Pointer table of the SoA state (all arrays n_pad long).
+struct Soa {
+    double *r[3];    // corrected position
+    double *m;       // mass as uploaded (double; energy uses it)
+    double *v[3];    // corrected velocity
+    double *f[6];    // acceleration a[3], jerk a1[3]
+    double *old[6];  // forces at the start of the step
+    double *a2[3];   // 2nd derivative from the Hermite interpolant
+    double *a3[3];   // 3rd derivative
+    double *t;       // particle time
+    double *dt;      // particle block timestep
+    double *p[6];    // predicted r[3], v[3]
+    double *pm;      // predictor mass, float-rounded (Predictor::m)
+};
+
+// Device-side scalars of one block step.
+struct DevInfo {
+    uint32_t nact;                // active count (atomic compaction target)
+    uint32_t pad0;
+    unsigned long long tmin_key;  // order-preserving key of min(t+dt)
+};
+
+// Order-preserving map double -> uint64 for atomicMin.
+__host__ __device__ inline unsigned long long dkey(double x) {
+    unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__host__ __device__ inline double dkey_inv(unsigned long long k) {
+    unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __builtin_bit_cast(double, u);
+}
+
+// Work split of the pairwise update: nact i particles in tiles of BS, the
+// j range (n_jt tiles of JT) cut into S chunks so that n_it * S work items
+// keep `grid` workgroups busy when nact is small.
+struct Decomp {
+    uint32_t n_it;             // i tiles
+    uint32_t S;                // j chunks
+    uint32_t tiles_per_chunk;  // JT tiles per chunk
+    uint32_t n_items;          // n_it * S
+    uint32_t rows;             // n_it * BS: stride of one chunk in partials
+};
+
+__host__ __device__ inline Decomp decompose(uint32_t nact, uint32_t n_jt,
+                                            uint32_t grid,
+                                            uint32_t max_split) {
+    Decomp d;
+    d.n_it = (nact + BS - 1) / BS;
+    uint32_t S = 1;
+    if (d.n_it > 0 && d.n_it < grid) S = (grid + d.n_it - 1) / d.n_it;
+    if (S > max_split) S = max_split;
+    if (S > n_jt) S = n_jt;
+    if (S < 1) S = 1;
+    d.tiles_per_chunk = (n_jt + S - 1) / S;
+    d.S = d.tiles_per_chunk ? (n_jt + d.tiles_per_chunk - 1) / d.tiles_per_chunk : 1;
+    d.n_items = d.n_it * d.S;
+    d.rows = d.n_it * BS;
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// Pair terms
+// ---------------------------------------------------------------------------
+
+// Hermite4CPU::force_calculation (Hermite4CPU.cpp:55-82), operation for
+// operation: no contraction, IEEE sqrt and division, same association.
+__device__ __forceinline__ void pair_exact(const double pi[6], double xj,
+                                           double yj, double zj, double vxj,
+                                           double vyj, double vzj, double mj,
+                                           double e2, double fo[6]) {
+#pragma clang fp contract(off)
+    double rx = xj - pi[0];
+    double ry = yj - pi[1];
+    double rz = zj - pi[2];
+    double vx = vxj - pi[3];
+    double vy = vyj - pi[4];
+    double vz = vzj - pi[5];
+
+    double r2 = rx * rx + ry * ry + rz * rz + e2;
+    double rinv = 1.0 / sqrt(r2);
+    double r2inv = rinv * rinv;
+    double r3inv = r2inv * rinv;
+    double r5inv = r2inv * r3inv;
+    double mr3inv = r3inv * mj;
+    double mr5inv = r5inv * mj;
+
+    double rv = rx * vx + ry * vy + rz * vz;
+
+    fo[0] += (rx * mr3inv);
+    fo[1] += (ry * mr3inv);
+    fo[2] += (rz * mr3inv);
+    fo[3] += (vx * mr3inv - (3 * rv) * rx * mr5inv);
+    fo[4] += (vy * mr3inv - (3 * rv) * ry * mr5inv);
+    fo[5] += (vz * mr3inv - (3 * rv) * rz * mr5inv);
+}
+
+// 1/sqrt(x) for x > 0: hardware estimate y0 (v_rsq_f64), then one
+// third-order step of the binomial series (1 - e)^(-1/2) = 1 + e/2 +
+// 3e^2/8 + ..., e = 1 - x*y0^2, which triples the number of correct bits.
+__device__ __forceinline__ double rsqrt_fp64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x, y * y, 1.0);
+    double p = fma(e, 0.375, 0.5);
+    return fma(y * e, p, y);
+}
+
+// Same pair as pair_exact with FMA contraction and rsqrt_fp64:
+//   a  += m r / |r|^3
+//   j  += m/|r|^3 (v - 3 (r.v)/|r|^2 r)
+__device__ __forceinline__ void pair_fp64(const double pi[6], double xj,
+                                          double yj, double zj, double vxj,
+                                          double vyj, double vzj, double mj,
+                                          double e2, double fo[6]) {
+    double rx = xj - pi[0];
+    double ry = yj - pi[1];
+    double rz = zj - pi[2];
+    double vx = vxj - pi[3];
+    double vy = vyj - pi[4];
+    double vz = vzj - pi[5];
+    double r2 = fma(rx, rx, fma(ry, ry, fma(rz, rz, e2)));
+    double rinv = rsqrt_fp64(r2);
+    double rinv2 = rinv * rinv;
+    double mr3 = (mj * rinv) * rinv2;
+    double rv = fma(rx, vx, fma(ry, vy, rz * vz));
+    double c = (3.0 * rv) * rinv2;
+    fo[0] = fma(rx, mr3, fo[0]);
+    fo[1] = fma(ry, mr3, fo[1]);
+    fo[2] = fma(rz, mr3, fo[2]);
+    fo[3] = fma(mr3, fma(-c, rx, vx), fo[3]);
+    fo[4] = fma(mr3, fma(-c, ry, vy), fo[4]);
+    fo[5] = fma(mr3, fma(-c, rz, vz), fo[5]);
+}
+
+}  // namespace h4

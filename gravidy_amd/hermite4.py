@@ -1,0 +1,254 @@
+"""Host-side mirror of GraviDy's Hermite4 integrator interface over the
+HIP backend (libh4hip.so).
+
+`Hermite4HIP` keeps the reference's method names and call order
+(Hermite4.hpp:48-88, cpu/Hermite4CPU.hpp:51-69): init_acc_jrk, init_dt,
+find_particles_to_move, save_old_acc_jrk, predicted_pos_vel,
+update_acc_jrk, correction_pos_vel, next_integration_time, get_energy and
+integration().  All state stays on the GPU; host arrays are produced only
+by download().
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, PRECISIONS
+
+# common.hpp:122,129
+D_TIME_MIN = 1.1920928955078125e-07
+D_TIME_MAX = 0.125
+# common.hpp:114,119 (float 0.01)
+ETA_DEFAULT = float(np.float32(0.01))
+
+
+def softening_e2(eps):
+    """Softening squared as NbodySystem.cpp:50 computes it (float eps,
+    float product)."""
+    e = np.float32(eps)
+    return float(np.float32(e * e))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _opt(a, shape):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+class Hermite4HIP:
+    """One N-body system resident on the current HIP device.
+
+    r: (n,4) positions + mass (double4 h_r); v: (n,3|4) velocities.
+    e2: softening squared (use softening_e2(eps) for the reference's value).
+    """
+
+    def __init__(self, r, v, e2, eta=ETA_DEFAULT, precision="fp64",
+                 dt_min=D_TIME_MIN, dt_max=D_TIME_MAX):
+        self.lib = _lib.load()
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if r.ndim != 2 or r.shape[1] != 4:
+            raise ValueError("r must be (n,4): x, y, z, m")
+        n = r.shape[0]
+        if v.shape == (n, 3):
+            v = np.concatenate([v, np.zeros((n, 1))], axis=1)
+        if v.shape != (n, 4):
+            raise ValueError("v must be (n,3) or (n,4)")
+        self.n = n
+        self.e2 = float(e2)
+        self.eta = float(np.float32(eta))
+        self.dt_min = float(dt_min)
+        self.dt_max = float(dt_max)
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self._ctx = ctypes.c_void_p()
+        check("h4h_create", self.lib.h4h_create(ctypes.byref(self._ctx), n, self.e2,
+                                                self.eta, self.dt_min, self.dt_max, prec))
+        self.upload(r, v)
+
+    # -- lifetime --------------------------------------------------------
+    def close(self):
+        if self._ctx:
+            self.lib.h4h_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- configuration ---------------------------------------------------
+    def set_precision(self, precision):
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        check("h4h_set_precision", self.lib.h4h_set_precision(self._ctx, prec))
+
+    def set_launch(self, workgroups=0, max_jsplit=0):
+        check("h4h_set_launch", self.lib.h4h_set_launch(self._ctx, workgroups, max_jsplit))
+
+    def set_timing(self, enable=True):
+        check("h4h_set_timing", self.lib.h4h_set_timing(self._ctx, 1 if enable else 0))
+
+    def stats(self, reset=False):
+        s = _lib.H4HStats()
+        check("h4h_get_stats", self.lib.h4h_get_stats(self._ctx, ctypes.byref(s), int(reset)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    # -- transfers -------------------------------------------------------
+    def upload(self, r, v, f=None, t=None, dt=None):
+        n = self.n
+        r = _opt(r, (n, 4))
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if v.shape == (n, 3):
+            v = np.concatenate([v, np.zeros((n, 1))], axis=1)
+        v = _opt(v, (n, 4))
+        f = _opt(f, (n, 6))
+        t = _opt(t, (n,))
+        dt = _opt(dt, (n,))
+        check("h4h_upload", self.lib.h4h_upload(
+            self._ctx, _dp(r), _dp(v), None if f is None else _dp(f),
+            None if t is None else _dp(t), None if dt is None else _dp(dt)))
+
+    def download(self):
+        n = self.n
+        out = {
+            "r": np.zeros((n, 4)), "v": np.zeros((n, 4)), "f": np.zeros((n, 6)),
+            "t": np.zeros(n), "dt": np.zeros(n), "old": np.zeros((n, 6)),
+            "a2": np.zeros((n, 3)), "a3": np.zeros((n, 3)),
+        }
+        check("h4h_download", self.lib.h4h_download(
+            self._ctx, _dp(out["r"]), _dp(out["v"]), _dp(out["f"]), _dp(out["t"]),
+            _dp(out["dt"]), _dp(out["old"]), _dp(out["a2"]), _dp(out["a3"])))
+        return out
+
+    def download_predictor(self):
+        p = np.zeros((self.n, 6))
+        pm = np.zeros(self.n, np.float32)
+        check("h4h_download_predictor", self.lib.h4h_download_predictor(
+            self._ctx, _dp(p), pm.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return p, pm
+
+    def upload_predictor(self, p, pm):
+        p = _opt(p, (self.n, 6))
+        pm = np.ascontiguousarray(pm, dtype=np.float32)
+        check("h4h_upload_predictor", self.lib.h4h_upload_predictor(
+            self._ctx, _dp(p), pm.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def set_active(self, move):
+        move = np.ascontiguousarray(move, dtype=np.uint32)
+        check("h4h_set_active", self.lib.h4h_set_active(
+            self._ctx, move.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), move.size))
+        return move.size
+
+    def get_active(self):
+        move = np.zeros(self.n, np.uint32)
+        k = ctypes.c_uint32()
+        check("h4h_get_active", self.lib.h4h_get_active(
+            self._ctx, move.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.byref(k)))
+        return move[:k.value].copy()
+
+    # -- the Hermite4 interface --------------------------------------------
+    def init_acc_jrk(self):
+        check("h4h_init_acc_jrk", self.lib.h4h_init_acc_jrk(self._ctx))
+
+    def init_dt(self, atime=1.0e10, eta=ETA_DEFAULT, itime=0.0):
+        c = ctypes.c_double(atime)
+        check("h4h_init_dt", self.lib.h4h_init_dt(self._ctx, ctypes.byref(c),
+                                                  float(np.float32(eta)), float(itime)))
+        return c.value
+
+    def find_particles_to_move(self, itime):
+        k = ctypes.c_uint32()
+        check("h4h_find_particles_to_move",
+              self.lib.h4h_find_particles_to_move(self._ctx, float(itime), ctypes.byref(k)))
+        return k.value
+
+    def save_old_acc_jrk(self):
+        check("h4h_save_old_acc_jrk", self.lib.h4h_save_old_acc_jrk(self._ctx))
+
+    def predicted_pos_vel(self, itime):
+        check("h4h_predicted_pos_vel", self.lib.h4h_predicted_pos_vel(self._ctx, float(itime)))
+
+    def update_acc_jrk(self, nact):
+        check("h4h_update_acc_jrk", self.lib.h4h_update_acc_jrk(self._ctx, int(nact)))
+
+    def correction_pos_vel(self, itime, nact):
+        check("h4h_correction_pos_vel",
+              self.lib.h4h_correction_pos_vel(self._ctx, float(itime), int(nact)))
+
+    def next_integration_time(self):
+        a = ctypes.c_double()
+        check("h4h_next_integration_time",
+              self.lib.h4h_next_integration_time(self._ctx, ctypes.byref(a)))
+        return a.value
+
+    def step(self, itime):
+        """Fused block step; returns (nact, next time)."""
+        k = ctypes.c_uint32()
+        a = ctypes.c_double()
+        check("h4h_step", self.lib.h4h_step(self._ctx, float(itime), ctypes.byref(k),
+                                            ctypes.byref(a)))
+        return k.value, a.value
+
+    def get_energy(self):
+        ek = ctypes.c_double()
+        ep = ctypes.c_double()
+        check("h4h_get_energy", self.lib.h4h_get_energy(self._ctx, ctypes.byref(ek),
+                                                        ctypes.byref(ep)))
+        return ek.value, ep.value
+
+    def synchronize(self):
+        check("h4h_synchronize", self.lib.h4h_synchronize(self._ctx))
+
+    def integration(self, t_end, interval=0.125, eta_s=ETA_DEFAULT, max_steps=None,
+                    on_output=None):
+        """Hermite4CPU::integration's loop (Hermite4CPU_integration.cpp:38-130)
+        with every kernel on the GPU.  integration_time and interval_time are
+        float as in the reference (NbodySystem.hpp:69-70) and the interval
+        test is evaluated in float.  Returns the schedule (ITIME and nact per
+        block step) and the energies at t=0 and at every output time."""
+        t_end32 = np.float32(t_end)
+        iv32 = np.float32(interval)
+        atime = 1.0e10
+        itime = 0.0
+        self.init_acc_jrk()
+        atime = self.init_dt(atime, eta_s, itime)
+        ek, ep = self.get_energy()
+        e_time, e_val = [itime], [ek + ep]
+        if on_output:
+            on_output(self, itime)
+        steps_t, steps_n = [], []
+        output_factor = 1
+        interactions = 0
+        while itime < t_end32:
+            if max_steps is not None and len(steps_t) >= max_steps:
+                break
+            itime = atime
+            nact, atime = self.step(itime)
+            interactions += nact * self.n
+            steps_t.append(itime)
+            steps_n.append(nact)
+            if itime >= float(np.float32(iv32 * np.float32(output_factor))):
+                ek, ep = self.get_energy()
+                e_time.append(itime)
+                e_val.append(ek + ep)
+                if on_output:
+                    on_output(self, itime)
+                output_factor += 1
+        return {
+            "step_time": np.array(steps_t), "step_nact": np.array(steps_n, np.uint32),
+            "energy_time": np.array(e_time), "energy": np.array(e_val),
+            "interactions": interactions,
+        }

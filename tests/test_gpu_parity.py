@@ -1,0 +1,239 @@
+"""Parity of the HIP backend (through the C ABI) with the CPU oracle.
+
+FP64_EXACT must be bit-identical to the oracle (Hermite4CPU's rounding):
+forces, predictor, corrector, timesteps and whole-run schedules.  FP64 (the
+fast kernel: refined v_rsq_f64, FMA, j split) must stay within
+REL_TOL = 1e-13 per particle, |a_gpu - a_ref| / |a_ref| (and the same for
+the jerk), on state-injected steps.  Energies are compared to 1e-13
+relative (only the summation order differs).
+"""
+import numpy as np
+import pytest
+
+from gravidy_amd import Hermite4HIP, ic, softening_e2
+
+pytestmark = pytest.mark.gpu
+
+E2 = softening_e2(1e-4)
+REL_TOL = 1e-13
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+def rel_err(a, b):
+    num = np.linalg.norm(a - b, axis=1)
+    den = np.linalg.norm(b, axis=1)
+    den = np.where(den == 0, 1.0, den)
+    return float((num / den).max())
+
+
+def p6(r, v):
+    return np.concatenate([r[:, :3], v[:, :3]], axis=1)
+
+
+def pm_of(r):
+    return r[:, 3].astype(np.float32)
+
+
+def block_state(n, seed, T):
+    """Plummer state with random forces and a block-time layout in which
+    t_i + dt_i >= T for every i and equality holds for dt_i <= 2^-9."""
+    rng = np.random.default_rng(seed)
+    r, v = ic.plummer(n, seed)
+    f = rng.normal(size=(n, 6)) * 0.3
+    k = rng.integers(3, 13, size=n)
+    dt = 2.0 ** -k.astype(np.float64)
+    t = np.floor((T - 1e-12) / dt) * dt
+    return r, v, f, t, dt
+
+
+# --- initial forces -------------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 2, 255, 256, 1000, 1024, 3000])
+def test_init_acc_jrk_exact_is_bitwise(oracle, n):
+    r, v = ic.plummer(n, 100 + n)
+    with Hermite4HIP(r, v, E2, precision="exact") as h:
+        h.init_acc_jrk()
+        f = h.download()["f"]
+    fo = oracle.init_acc_jrk(p6(r, v), pm_of(r), E2)
+    assert bits_equal(f, fo)
+
+
+@pytest.mark.parametrize("n", [2, 255, 1024, 3000, 8192])
+def test_init_acc_jrk_fp64_within_tolerance(oracle, n):
+    r, v = ic.plummer(n, 200 + n)
+    with Hermite4HIP(r, v, E2, precision="fp64") as h:
+        h.init_acc_jrk()
+        f = h.download()["f"]
+    fo = oracle.init_acc_jrk(p6(r, v), pm_of(r), E2)
+    ea, ej = rel_err(f[:, :3], fo[:, :3]), rel_err(f[:, 3:], fo[:, 3:])
+    print(f"n={n} fp64 max rel err acc {ea:.3e} jerk {ej:.3e}")
+    assert ea <= REL_TOL and ej <= REL_TOL
+
+
+def test_init_acc_jrk_without_softening(oracle):
+    r, v = ic.plummer(512, 77)
+    fo = oracle.init_acc_jrk(p6(r, v), pm_of(r), 0.0)
+    for prec in ("exact", "fp64"):
+        with Hermite4HIP(r, v, 0.0, precision=prec) as h:
+            h.init_acc_jrk()
+            f = h.download()["f"]
+        assert np.all(np.isfinite(f))
+        if prec == "exact":
+            assert bits_equal(f, fo)
+        else:
+            assert rel_err(f[:, :3], fo[:, :3]) <= REL_TOL
+
+
+# --- one state-injected block step ---------------------------------------
+
+@pytest.mark.parametrize("n,seed", [(1024, 1), (1000, 2), (4096, 3)])
+def test_block_step_pieces_exact_are_bitwise(oracle, n, seed):
+    T = 0.375 + 2.0 ** -9
+    r, v, f, t, dt = block_state(n, seed, T)
+    with Hermite4HIP(r, v, E2, precision="exact") as h:
+        h.upload(r, v, f, t, dt)
+        nact = h.find_particles_to_move(T)
+        move = h.get_active()
+        mo = oracle.find_particles_to_move(T, t, dt)
+        assert nact == mo.size > 0
+        assert np.array_equal(move, mo)
+
+        h.save_old_acc_jrk()
+        h.predicted_pos_vel(T)
+        p, pm = h.download_predictor()
+        po, pmo = oracle.predicted_pos_vel(T, t, r, v, f)
+        assert bits_equal(p, po) and bits_equal(pm, pmo)
+
+        h.update_acc_jrk(nact)
+        fo = oracle.update_acc_jrk(mo, po, pmo, E2, f)
+        h.correction_pos_vel(T, nact)
+        atime = h.next_integration_time()
+        s = h.download()
+        a2o, a3o, ro, vo, to, dto = oracle.correction_pos_vel(
+            T, mo, po, fo, f, np.zeros((n, 3)), np.zeros((n, 3)), r, v, t, dt)
+        assert bits_equal(s["f"], fo)
+        assert bits_equal(s["old"][mo], f[mo])
+        assert bits_equal(s["a2"][mo], a2o[mo]) and bits_equal(s["a3"][mo], a3o[mo])
+        assert bits_equal(s["r"], ro) and bits_equal(s["v"][:, :3], vo[:, :3])
+        assert bits_equal(s["t"], to) and bits_equal(s["dt"], dto)
+        assert atime == oracle.next_integration_time(to, dto)
+
+
+@pytest.mark.parametrize("n,nact", [(1024, 1), (1024, 37), (4096, 700), (65536, 1),
+                                    (65536, 37), (65536, 300), (65536, 5000)])
+def test_update_fp64_jsplit_within_tolerance(oracle, n, nact):
+    rng = np.random.default_rng(n + nact)
+    r, v = ic.plummer(n, 300 + n)
+    move = np.sort(rng.choice(n, size=nact, replace=False)).astype(np.uint32)
+    f0 = np.zeros((n, 6))
+    with Hermite4HIP(r, v, E2, precision="fp64") as h:
+        h.set_active(move)
+        h.update_acc_jrk(nact)
+        f = h.download()["f"]
+    fo = oracle.update_acc_jrk(move, p6(r, v), pm_of(r), E2, f0)
+    ea = rel_err(f[move, :3], fo[move, :3])
+    ej = rel_err(f[move, 3:], fo[move, 3:])
+    print(f"n={n} nact={nact} fp64 max rel err acc {ea:.3e} jerk {ej:.3e}")
+    assert ea <= REL_TOL and ej <= REL_TOL
+    rest = np.setdiff1d(np.arange(n), move)
+    assert np.all(f[rest] == 0.0)
+
+
+# --- initial timesteps ----------------------------------------------------
+
+def _exact_power_inputs():
+    """|a| values that make ETA*(|a|/1) land exactly on, and one ulp either
+    side of, powers of two (|j| = 1)."""
+    eta = float(np.float32(0.01))
+    rows = []
+    for k in range(-12, -2):
+        target = 2.0 ** k
+        x = target / eta
+        for _ in range(64):
+            y = eta * x
+            if y == target:
+                break
+            x = np.nextafter(x, np.inf if y < target else -np.inf)
+        for a in (x, np.nextafter(x, np.inf), np.nextafter(x, -np.inf)):
+            rows.append([a, 0, 0, 1.0, 0, 0])
+    rows.append([1.0, 0, 0, 0, 0, 0])   # |j| = 0
+    rows.append([0, 0, 0, 1.0, 0, 0])   # |a| = 0
+    rows.append([0, 0, 0, 0, 0, 0])     # 0/0
+    return np.array(rows, dtype=np.float64)
+
+
+def test_init_dt_is_bitwise(oracle):
+    r, v = ic.plummer(2048, 5)
+    fo = oracle.init_acc_jrk(p6(r, v), pm_of(r), E2)
+    extra = _exact_power_inputs()
+    f = np.concatenate([fo, extra])
+    n = f.shape[0]
+    r2 = np.concatenate([r, np.tile(r[:1], (extra.shape[0], 1))])
+    v2 = np.concatenate([v, np.tile(v[:1], (extra.shape[0], 1))])
+    with Hermite4HIP(r2, v2, E2, precision="exact") as h:
+        h.upload(r2, v2, f)
+        ctime = h.init_dt(1.0e10, itime=0.0)
+        s = h.download()
+    to, dto, co = oracle.init_dt(f, itime=0.0)
+    bad = np.nonzero(s["dt"] != dto)[0]
+    assert bad.size == 0, [(i, f[i], s["dt"][i], dto[i]) for i in bad[:5]]
+    assert bits_equal(s["t"], to)
+    assert ctime == co
+
+
+# --- energy ---------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 2, 8, 1000, 4096])
+def test_energy_matches(oracle, n):
+    if n == 8:
+        r = np.array([[x, y, z, 1.0] for x in (0, 1) for y in (0, 1) for z in (0, 1)], float)
+        v = np.zeros((8, 4))
+    else:
+        r, v = ic.plummer(n, 400 + n)
+    with Hermite4HIP(r, v, E2) as h:
+        ek, ep = h.get_energy()
+    eko, epo = oracle.get_energy(r, v)
+    assert ek == pytest.approx(eko, rel=1e-13, abs=1e-300)
+    assert ep == pytest.approx(epo, rel=1e-13, abs=1e-300)
+
+
+# --- whole runs -------------------------------------------------------------
+
+def test_c1_run_exact_is_bitwise(oracle):
+    """Plummer N=1024, eps=1e-4, eta=0.01, dt in [2^-23, 2^-3], to t=1:
+    identical block-timestep schedule and bit-identical final state."""
+    r, v = ic.plummer(1024, 12345)
+    ref = oracle.integrate(r, v, E2, 1.0)
+    with Hermite4HIP(r, v, E2, precision="exact") as h:
+        run = h.integration(1.0)
+        s = h.download()
+    assert np.array_equal(run["step_time"], ref["step_time"])
+    assert np.array_equal(run["step_nact"], ref["step_nact"])
+    assert run["interactions"] == ref["interactions"]
+    assert bits_equal(s["r"], ref["r"]) and bits_equal(s["v"][:, :3], ref["v"][:, :3])
+    assert bits_equal(s["dt"], ref["dt"]) and bits_equal(s["t"], ref["t"])
+    assert np.array_equal(run["energy_time"], ref["energy_time"])
+    np.testing.assert_allclose(run["energy"], ref["energy"], rtol=1e-13)
+
+
+def test_c1_run_fp64_energy_error_matches(oracle):
+    """Fast FP64 kernel over the same run: the cumulative energy error at
+    t_end (CumE, Logger.cpp:328) agrees with the oracle's to 2 significant
+    digits; the schedule agreement is reported."""
+    r, v = ic.plummer(1024, 12345)
+    ref = oracle.integrate(r, v, E2, 1.0)
+    with Hermite4HIP(r, v, E2, precision="fp64") as h:
+        run = h.integration(1.0)
+    same = (run["step_time"].shape == ref["step_time"].shape
+            and np.array_equal(run["step_time"], ref["step_time"])
+            and np.array_equal(run["step_nact"], ref["step_nact"]))
+    cum = abs((run["energy"][-1] - run["energy"][0]) / run["energy"][0])
+    cum_ref = abs((ref["energy"][-1] - ref["energy"][0]) / ref["energy"][0])
+    print(f"fp64 C1: schedule identical={same} steps={len(run['step_time'])}/"
+          f"{len(ref['step_time'])} CumE={cum:.6e} oracle CumE={cum_ref:.6e}")
+    assert float(f"{cum:.1e}") == float(f"{cum_ref:.1e}") or abs(cum - cum_ref) <= 0.05 * cum_ref
