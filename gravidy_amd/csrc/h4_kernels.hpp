@@ -55,32 +55,54 @@ __host__ __device__ inline double dkey_inv(unsigned long long k) {
     return __builtin_bit_cast(double, u);
 }
 
-// Work split of the pairwise update: nact i particles in tiles of BS, the
-// j range (n_jt tiles of JT) cut into S chunks so that n_it * S work items
-// keep `grid` workgroups busy when nact is small.
+// Work split of the pairwise update.  A workgroup handles IW active
+// particles ("i slots") at a time; its BS lanes are IW slots x BS/IW j
+// sub-groups.  The nact active particles form n_it = ceil(nact/IW) i tiles
+// and the j range (n_jt tiles of JT) is cut into S chunks so that
+// n_it * S work items keep `grid` workgroups busy when nact is small.
+// A chunk holds at least 64/IW tiles so one item is >= 16K pairs.
 struct Decomp {
     uint32_t n_it;             // i tiles
     uint32_t S;                // j chunks
     uint32_t tiles_per_chunk;  // JT tiles per chunk
     uint32_t n_items;          // n_it * S
-    uint32_t rows;             // n_it * BS: stride of one chunk in partials
+    uint32_t rows;             // n_it * IW: stride of one chunk in partials
 };
 
-__host__ __device__ inline Decomp decompose(uint32_t nact, uint32_t n_jt,
-                                            uint32_t grid,
+__host__ __device__ inline Decomp decompose(uint32_t nact, uint32_t iw,
+                                            uint32_t n_jt, uint32_t grid,
                                             uint32_t max_split) {
     Decomp d;
-    d.n_it = (nact + BS - 1) / BS;
+    d.n_it = (nact + iw - 1) / iw;
+    uint32_t min_tiles = iw >= 64 ? 1u : 64u / iw;
     uint32_t S = 1;
     if (d.n_it > 0 && d.n_it < grid) S = (grid + d.n_it - 1) / d.n_it;
     if (S > max_split) S = max_split;
-    if (S > n_jt) S = n_jt;
+    uint32_t s_cap = (n_jt + min_tiles - 1) / min_tiles;
+    if (S > s_cap) S = s_cap;
     if (S < 1) S = 1;
     d.tiles_per_chunk = (n_jt + S - 1) / S;
     d.S = d.tiles_per_chunk ? (n_jt + d.tiles_per_chunk - 1) / d.tiles_per_chunk : 1;
     d.n_items = d.n_it * d.S;
-    d.rows = d.n_it * BS;
+    d.rows = d.n_it * iw;
     return d;
+}
+
+// Lane-group width for nact active particles: the widest of 256/64/16/4
+// that keeps at least 3/4 of the i slots busy, else the least padded.
+__host__ __device__ inline uint32_t pick_iw(uint32_t nact) {
+    const uint32_t opts[4] = {256, 64, 16, 4};
+    uint32_t best = 256, best_pad = 0xffffffffu;
+    for (int k = 0; k < 4; ++k) {
+        uint32_t iw = opts[k];
+        uint32_t slots = (nact + iw - 1) / iw * iw;
+        if (4ull * nact >= 3ull * slots) return iw;
+        if (slots < best_pad) {
+            best_pad = slots;
+            best = iw;
+        }
+    }
+    return best;
 }
 
 // ---------------------------------------------------------------------------

@@ -226,15 +226,19 @@ __global__ __launch_bounds__(BS) void k_update_exact(Soa s,
     }
 }
 
-// Inner loop over one LDS tile.  SELF: the tile may hold the lane's own i,
-// whose term is skipped (mass 0, r2 = 1 keeps it finite when e2 == 0).
-template <bool SELF>
+// Inner loop of one lane over one LDS tile: the lane's j sub-group js
+// takes tile entries js, js + G, ..., G = BS/IW (all JT entries when
+// IW = BS).  SELF: the tile may hold the lane's own i, whose term is
+// skipped (mass 0, and r2 kept finite when e2 == 0).
+template <int IW, bool SELF>
 __device__ __forceinline__ void tile_fp64(const double pi[6], uint32_t i,
-                                          uint32_t j0, const double4 *sp,
-                                          const double4 *sv, double e2,
-                                          double fo[6]) {
+                                          uint32_t j0, uint32_t js,
+                                          const double4 *sp, const double4 *sv,
+                                          double e2, double fo[6]) {
+    constexpr uint32_t G = BS / IW;
 #pragma unroll 4
-    for (uint32_t jj = 0; jj < (uint32_t)JT; ++jj) {
+    for (uint32_t q = 0; q < (uint32_t)JT / G; ++q) {
+        uint32_t jj = js + q * G;
         double4 a = sp[jj];
         double4 b = sv[jj];
         if (SELF) {
@@ -248,9 +252,45 @@ __device__ __forceinline__ void tile_fp64(const double pi[6], uint32_t i,
     }
 }
 
+// Sums fo over the BS/IW j sub-groups of each i slot: shuffles across the
+// sub-groups inside a wave (lane bits >= log2(IW)), then LDS across the
+// BS/64 waves in wave order.  The fixed tree makes it deterministic.  On
+// return lanes threadIdx.x < IW hold the sums of slot threadIdx.x.
+template <int IW>
+__device__ __forceinline__ void reduce_subgroups(double fo[6], double *red) {
+    if constexpr (IW < 64) {
+#pragma unroll
+        for (int o = IW; o < 64; o <<= 1)
+            for (int c = 0; c < 6; ++c) fo[c] += __shfl_xor(fo[c], o);
+    }
+    if constexpr (IW < BS) {
+        constexpr uint32_t W = BS / 64;            // waves
+        constexpr uint32_t SL = IW < 64 ? IW : 64;  // slots held per wave
+        uint32_t lane = __lane_id();
+        uint32_t w = threadIdx.x >> 6;
+        __syncthreads();
+        if (lane < SL) {
+            // wave w holds slots (w*64 .. w*64+63) % IW when IW >= 64
+            uint32_t slot = (IW >= 64) ? (threadIdx.x % IW) : lane;
+            uint32_t grp = (IW >= 64) ? (threadIdx.x / IW) : w;
+            for (int c = 0; c < 6; ++c) red[(grp * IW + slot) * 6 + c] = fo[c];
+        }
+        __syncthreads();
+        if (threadIdx.x < IW) {
+            constexpr uint32_t NG = (IW >= 64) ? BS / IW : W;
+            for (int c = 0; c < 6; ++c) {
+                double acc = red[threadIdx.x * 6 + c];
+                for (uint32_t g = 1; g < NG; ++g) acc += red[(g * IW + threadIdx.x) * 6 + c];
+                fo[c] = acc;
+            }
+        }
+    }
+}
+
 // FP64: persistent workgroups over (i tile, j chunk) items.  With one
 // chunk the result goes to f[i]; with several, chunk partial sums go to
 // partial[c][chunk][k] and k_reduce_partials adds them in chunk order.
+template <int IW>
 __global__ __launch_bounds__(BS) void k_update_fp64(Soa s,
                                                    const uint32_t *move,
                                                    const DevInfo *info,
@@ -260,12 +300,15 @@ __global__ __launch_bounds__(BS) void k_update_fp64(Soa s,
                                                    double e2) {
     __shared__ double4 sp[JT];
     __shared__ double4 sv[JT];
+    __shared__ double red[(IW < BS ? BS : 1) * 6];
     const uint32_t nact = info->nact;
-    const Decomp d = decompose(nact, n_jt, gridDim.x, max_split);
+    const Decomp d = decompose(nact, IW, n_jt, gridDim.x, max_split);
+    const uint32_t il = threadIdx.x % IW;
+    const uint32_t js = threadIdx.x / IW;
     for (uint32_t item = blockIdx.x; item < d.n_items; item += gridDim.x) {
         uint32_t it = item / d.S;
         uint32_t sc = item - it * d.S;
-        uint32_t k = it * BS + threadIdx.x;
+        uint32_t k = it * IW + il;
         bool act = k < nact;
         uint32_t i = act ? move[k] : move[0];
         double pi[6];
@@ -280,11 +323,12 @@ __global__ __launch_bounds__(BS) void k_update_fp64(Soa s,
             __syncthreads();
             bool self = (i - j0) < (uint32_t)JT;
             if (__ballot(self))
-                tile_fp64<true>(pi, i, j0, sp, sv, e2, fo);
+                tile_fp64<IW, true>(pi, i, j0, js, sp, sv, e2, fo);
             else
-                tile_fp64<false>(pi, i, j0, sp, sv, e2, fo);
+                tile_fp64<IW, false>(pi, i, j0, js, sp, sv, e2, fo);
         }
-        if (act) {
+        reduce_subgroups<IW>(fo, red);
+        if (threadIdx.x < IW && act) {
             if (d.S == 1) {
                 for (int c = 0; c < 6; ++c) s.f[c][i] = fo[c];
             } else {
@@ -296,25 +340,49 @@ __global__ __launch_bounds__(BS) void k_update_fp64(Soa s,
     }
 }
 
+// Adds the S chunk partials of every active particle in chunk order.
+// Few chunks: one lane per particle.  Many chunks: one workgroup per
+// particle, lanes striding over chunks, then a fixed-order tree.
 __global__ __launch_bounds__(BS) void k_reduce_partials(Soa s,
                                                        const uint32_t *move,
                                                        const DevInfo *info,
                                                        const double *partial,
+                                                       uint32_t iw,
                                                        uint32_t n_jt,
                                                        uint32_t grid,
                                                        uint32_t max_split) {
+    __shared__ double red[BS / 64];
     const uint32_t nact = info->nact;
-    const Decomp d = decompose(nact, n_jt, grid, max_split);
+    const Decomp d = decompose(nact, iw, n_jt, grid, max_split);
     if (d.S == 1) return;
     size_t plane = (size_t)d.S * d.rows;
-    for (uint32_t k = blockIdx.x * BS + threadIdx.x; k < nact;
-         k += gridDim.x * BS) {
+    if (d.S <= 32) {
+        for (uint32_t k = blockIdx.x * BS + threadIdx.x; k < nact; k += gridDim.x * BS) {
+            uint32_t i = move[k];
+            for (int c = 0; c < 6; ++c) {
+                double acc = 0.0;
+                for (uint32_t sc = 0; sc < d.S; ++sc)
+                    acc += partial[c * plane + (size_t)sc * d.rows + k];
+                s.f[c][i] = acc;
+            }
+        }
+        return;
+    }
+    for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
         uint32_t i = move[k];
         for (int c = 0; c < 6; ++c) {
             double acc = 0.0;
-            for (uint32_t sc = 0; sc < d.S; ++sc)
+            for (uint32_t sc = threadIdx.x; sc < d.S; sc += BS)
                 acc += partial[c * plane + (size_t)sc * d.rows + k];
-            s.f[c][i] = acc;
+            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+            __syncthreads();
+            if (__lane_id() == 0) red[threadIdx.x >> 6] = acc;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double t = red[0];
+                for (uint32_t w = 1; w < BS / 64; ++w) t += red[w];
+                s.f[c][i] = t;
+            }
         }
     }
 }
@@ -638,15 +706,24 @@ static int update_pass(h4h_ctx *ctx) {
         H4H_LAUNCHED();
     } else {
         // H4H_MIXED runs the FP64 pair kernel until its own kernel exists.
-        hipLaunchKernelGGL(k_update_fp64, dim3(ctx->grid), dim3(BS), 0, ctx->stream, ctx->s,
+        uint32_t iw = pick_iw(ctx->nact_host);
+        auto *kern = k_update_fp64<256>;
+        if (iw == 64) kern = k_update_fp64<64>;
+        else if (iw == 16) kern = k_update_fp64<16>;
+        else if (iw == 4) kern = k_update_fp64<4>;
+        hipLaunchKernelGGL(kern, dim3(ctx->grid), dim3(BS), 0, ctx->stream, ctx->s,
                            ctx->move, ctx->info, ctx->partial, ctx->n_jt, ctx->max_split,
                            ctx->e2);
         H4H_LAUNCHED();
-        uint32_t rgrid = std::max<uint32_t>(1, std::min<uint32_t>(1024, blocks_for(ctx->nact_host)));
-        hipLaunchKernelGGL(k_reduce_partials, dim3(rgrid), dim3(BS), 0, ctx->stream, ctx->s,
-                           ctx->move, ctx->info, ctx->partial, ctx->n_jt, ctx->grid,
-                           ctx->max_split);
-        H4H_LAUNCHED();
+        Decomp d = decompose(ctx->nact_host, iw, ctx->n_jt, ctx->grid, ctx->max_split);
+        if (d.S > 1) {
+            uint32_t rgrid = d.S <= 32 ? std::max<uint32_t>(1, std::min<uint32_t>(1024, blocks_for(ctx->nact_host)))
+                                       : std::min<uint32_t>(ctx->nact_host, 2048);
+            hipLaunchKernelGGL(k_reduce_partials, dim3(rgrid), dim3(BS), 0, ctx->stream, ctx->s,
+                               ctx->move, ctx->info, ctx->partial, iw, ctx->n_jt, ctx->grid,
+                               ctx->max_split);
+            H4H_LAUNCHED();
+        }
     }
     if (ctx->timing) {
         H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));

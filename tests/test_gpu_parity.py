@@ -106,6 +106,21 @@ def test_init_acc_jrk_fp64_within_tolerance(oracle, n):
     check_fp64(f, fo, np.arange(n), p6(r, v), pm_of(r), E2, f"init n={n} fp64")
 
 
+@pytest.mark.parametrize("nact", [1, 10, 37, 100])
+def test_update_fp64_small_active_sets_without_softening(oracle, nact):
+    n = 4096
+    rng = np.random.default_rng(nact)
+    r, v = ic.plummer(n, 55)
+    move = np.sort(rng.choice(n, size=nact, replace=False)).astype(np.uint32)
+    with Hermite4HIP(r, v, 0.0, precision="fp64") as h:
+        h.set_active(move)
+        h.update_acc_jrk(nact)
+        f = h.download()["f"]
+    fo = oracle.update_acc_jrk(move, p6(r, v), pm_of(r), 0.0, np.zeros((n, 6)))
+    assert np.all(np.isfinite(f))
+    check_fp64(f, fo, move, p6(r, v), pm_of(r), 0.0, f"update e2=0 nact={nact} fp64")
+
+
 def test_init_acc_jrk_without_softening(oracle):
     r, v = ic.plummer(512, 77)
     fo = oracle.init_acc_jrk(p6(r, v), pm_of(r), 0.0)
@@ -155,8 +170,11 @@ def test_block_step_pieces_exact_are_bitwise(oracle, n, seed):
         assert atime == oracle.next_integration_time(to, dto)
 
 
+# nact picks the lane-group width: 1, 3, 10 -> 4; 37 -> 16; 100, 300 -> 64;
+# 700, 5000 -> 256 i slots per workgroup.
 @pytest.mark.parametrize("n,nact", [(1024, 1), (1024, 37), (4096, 700), (65536, 1),
-                                    (65536, 37), (65536, 300), (65536, 5000)])
+                                    (65536, 3), (65536, 10), (65536, 37), (65536, 100),
+                                    (65536, 300), (65536, 5000), (300000, 64)])
 def test_update_fp64_jsplit_within_tolerance(oracle, n, nact):
     rng = np.random.default_rng(n + nact)
     r, v = ic.plummer(n, 300 + n)
