@@ -178,4 +178,40 @@ __device__ __forceinline__ void pair_fp64(const double pi[6], double xj,
     fo[5] = fma(mr3, fma(-c, rz, vz), fo[5]);
 }
 
+// 1/sqrt(x) in FP32: v_rsq_f32 estimate and one Newton step
+// y (1 + e/2), e = 1 - x y^2.
+__device__ __forceinline__ float rsqrt_fp32(float x) {
+    float y = __builtin_amdgcn_rsqf(x);
+    float e = fmaf(-x, y * y, 1.0f);
+    return fmaf(y * 0.5f, e, y);
+}
+
+// Mixed-precision pair: position and velocity differences in FP64 (they
+// cancel for close pairs), rounded once to FP32; r^2, 1/r, the force and
+// jerk terms in FP32, added into the FP32 partial sums t[6].  The caller
+// folds t into FP64 accumulators every few pairs.
+__device__ __forceinline__ void pair_mixed(const double pi[6], double xj,
+                                           double yj, double zj, double vxj,
+                                           double vyj, double vzj, float mj,
+                                           float e2, float t[6]) {
+    float rx = (float)(xj - pi[0]);
+    float ry = (float)(yj - pi[1]);
+    float rz = (float)(zj - pi[2]);
+    float vx = (float)(vxj - pi[3]);
+    float vy = (float)(vyj - pi[4]);
+    float vz = (float)(vzj - pi[5]);
+    float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, e2)));
+    float rinv = rsqrt_fp32(r2);
+    float rinv2 = rinv * rinv;
+    float mr3 = (mj * rinv) * rinv2;
+    float rv = fmaf(rx, vx, fmaf(ry, vy, rz * vz));
+    float c = (3.0f * rv) * rinv2;
+    t[0] = fmaf(rx, mr3, t[0]);
+    t[1] = fmaf(ry, mr3, t[1]);
+    t[2] = fmaf(rz, mr3, t[2]);
+    t[3] = fmaf(mr3, fmaf(-c, rx, vx), t[3]);
+    t[4] = fmaf(mr3, fmaf(-c, ry, vy), t[4]);
+    t[5] = fmaf(mr3, fmaf(-c, rz, vz), t[5]);
+}
+
 }  // namespace h4

@@ -184,11 +184,20 @@ __global__ __launch_bounds__(BS) void k_save_old(Soa s, const uint32_t *move,
 
 // Loads JT consecutive predictor records starting at j0 into LDS; each of
 // the BS (= JT) threads moves one record with coalesced SoA loads.
+// The fourth slot of sv carries the mass as a float bit pattern in its low
+// 32 bits, so the mixed-precision kernel reads it without a conversion.
 __device__ __forceinline__ void load_jtile(const Soa &s, uint32_t j0,
                                            double4 *sp, double4 *sv) {
     uint32_t j = j0 + threadIdx.x;
-    sp[threadIdx.x] = make_double4(s.p[0][j], s.p[1][j], s.p[2][j], s.pm[j]);
-    sv[threadIdx.x] = make_double4(s.p[3][j], s.p[4][j], s.p[5][j], 0.0);
+    double m = s.pm[j];
+    unsigned long long mf = __builtin_bit_cast(uint32_t, (float)m);
+    sp[threadIdx.x] = make_double4(s.p[0][j], s.p[1][j], s.p[2][j], m);
+    sv[threadIdx.x] = make_double4(s.p[3][j], s.p[4][j], s.p[5][j],
+                                   __builtin_bit_cast(double, mf));
+}
+
+__device__ __forceinline__ float mass_f32(const double4 &b) {
+    return __builtin_bit_cast(float, (uint32_t)__builtin_bit_cast(unsigned long long, b.w));
 }
 
 // FP64_EXACT: one lane per active i, j = 0..n-1 ascending, j == i skipped,
@@ -229,25 +238,51 @@ __global__ __launch_bounds__(BS) void k_update_exact(Soa s,
 // Inner loop of one lane over one LDS tile: the lane's j sub-group js
 // takes tile entries js, js + G, ..., G = BS/IW (all JT entries when
 // IW = BS).  SELF: the tile may hold the lane's own i, whose term is
-// skipped (mass 0, and r2 kept finite when e2 == 0).
-template <int IW, bool SELF>
-__device__ __forceinline__ void tile_fp64(const double pi[6], uint32_t i,
-                                          uint32_t j0, uint32_t js,
-                                          const double4 *sp, const double4 *sv,
-                                          double e2, double fo[6]) {
+// skipped (mass 0, and r2 kept finite when e2 == 0).  MIXED: FP32 pair
+// terms summed in FP32 over runs of at most 32 pairs, each run then added
+// to the FP64 accumulators.
+template <int IW, bool SELF, bool MIXED>
+__device__ __forceinline__ void tile_pairs(const double pi[6], uint32_t i,
+                                           uint32_t j0, uint32_t js,
+                                           const double4 *sp, const double4 *sv,
+                                           double e2, double fo[6]) {
     constexpr uint32_t G = BS / IW;
+    constexpr uint32_t NQ = JT / G;
+    if constexpr (!MIXED) {
 #pragma unroll 4
-    for (uint32_t q = 0; q < (uint32_t)JT / G; ++q) {
-        uint32_t jj = js + q * G;
-        double4 a = sp[jj];
-        double4 b = sv[jj];
-        if (SELF) {
-            bool me = (j0 + jj) == i;
-            double mj = me ? 0.0 : a.w;
-            double x = me ? pi[0] + 1.0 : a.x;
-            pair_fp64(pi, x, a.y, a.z, b.x, b.y, b.z, mj, e2, fo);
-        } else {
-            pair_fp64(pi, a.x, a.y, a.z, b.x, b.y, b.z, a.w, e2, fo);
+        for (uint32_t q = 0; q < NQ; ++q) {
+            uint32_t jj = js + q * G;
+            double4 a = sp[jj];
+            double4 b = sv[jj];
+            if (SELF) {
+                bool me = (j0 + jj) == i;
+                double mj = me ? 0.0 : a.w;
+                double x = me ? pi[0] + 1.0 : a.x;
+                pair_fp64(pi, x, a.y, a.z, b.x, b.y, b.z, mj, e2, fo);
+            } else {
+                pair_fp64(pi, a.x, a.y, a.z, b.x, b.y, b.z, a.w, e2, fo);
+            }
+        }
+    } else {
+        constexpr uint32_t FOLD = NQ < 32 ? NQ : 32;
+        const float e2f = (float)e2;
+        for (uint32_t q0 = 0; q0 < NQ; q0 += FOLD) {
+            float t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+            for (uint32_t q = q0; q < q0 + FOLD; ++q) {
+                uint32_t jj = js + q * G;
+                double4 a = sp[jj];
+                double4 b = sv[jj];
+                if (SELF) {
+                    bool me = (j0 + jj) == i;
+                    float mj = me ? 0.f : mass_f32(b);
+                    double x = me ? pi[0] + 1.0 : a.x;
+                    pair_mixed(pi, x, a.y, a.z, b.x, b.y, b.z, mj, e2f, t);
+                } else {
+                    pair_mixed(pi, a.x, a.y, a.z, b.x, b.y, b.z, mass_f32(b), e2f, t);
+                }
+            }
+            for (int c = 0; c < 6; ++c) fo[c] += (double)t[c];
         }
     }
 }
@@ -287,10 +322,11 @@ __device__ __forceinline__ void reduce_subgroups(double fo[6], double *red) {
     }
 }
 
-// FP64: persistent workgroups over (i tile, j chunk) items.  With one
-// chunk the result goes to f[i]; with several, chunk partial sums go to
-// partial[c][chunk][k] and k_reduce_partials adds them in chunk order.
-template <int IW>
+// FP64 and mixed precision: persistent workgroups over (i tile, j chunk)
+// items.  With one chunk the result goes to f[i]; with several, chunk
+// partial sums go to partial[c][chunk][k] and k_reduce_partials adds them
+// in chunk order.
+template <int IW, bool MIXED>
 __global__ __launch_bounds__(BS) void k_update_fp64(Soa s,
                                                    const uint32_t *move,
                                                    const DevInfo *info,
@@ -323,9 +359,9 @@ __global__ __launch_bounds__(BS) void k_update_fp64(Soa s,
             __syncthreads();
             bool self = (i - j0) < (uint32_t)JT;
             if (__ballot(self))
-                tile_fp64<IW, true>(pi, i, j0, js, sp, sv, e2, fo);
+                tile_pairs<IW, true, MIXED>(pi, i, j0, js, sp, sv, e2, fo);
             else
-                tile_fp64<IW, false>(pi, i, j0, js, sp, sv, e2, fo);
+                tile_pairs<IW, false, MIXED>(pi, i, j0, js, sp, sv, e2, fo);
         }
         reduce_subgroups<IW>(fo, red);
         if (threadIdx.x < IW && act) {
@@ -705,12 +741,12 @@ static int update_pass(h4h_ctx *ctx) {
                            ctx->move, ctx->info, ctx->n, ctx->e2);
         H4H_LAUNCHED();
     } else {
-        // H4H_MIXED runs the FP64 pair kernel until its own kernel exists.
         uint32_t iw = pick_iw(ctx->nact_host);
-        auto *kern = k_update_fp64<256>;
-        if (iw == 64) kern = k_update_fp64<64>;
-        else if (iw == 16) kern = k_update_fp64<16>;
-        else if (iw == 4) kern = k_update_fp64<4>;
+        bool mixed = ctx->precision == H4H_MIXED;
+        auto *kern = mixed ? k_update_fp64<256, true> : k_update_fp64<256, false>;
+        if (iw == 64) kern = mixed ? k_update_fp64<64, true> : k_update_fp64<64, false>;
+        else if (iw == 16) kern = mixed ? k_update_fp64<16, true> : k_update_fp64<16, false>;
+        else if (iw == 4) kern = mixed ? k_update_fp64<4, true> : k_update_fp64<4, false>;
         hipLaunchKernelGGL(kern, dim3(ctx->grid), dim3(BS), 0, ctx->stream, ctx->s,
                            ctx->move, ctx->info, ctx->partial, ctx->n_jt, ctx->max_split,
                            ctx->e2);
