@@ -41,6 +41,12 @@ int fail(int code, const std::string &msg) {
                                       hipGetErrorString(e_));                \
     } while (0)
 
+// FP32 partial sums of the mixed kernel are folded into FP64 every
+// H4_MIXED_FOLD pairs per lane.
+#ifndef H4_MIXED_FOLD
+#define H4_MIXED_FOLD 32
+#endif
+
 constexpr uint32_t DEFAULT_GRID = 2048;       // persistent workgroups
 constexpr uint32_t DEFAULT_MAX_SPLIT = 1024;  // largest j split
 constexpr uint32_t ENERGY_GRID = 1024;
@@ -264,7 +270,7 @@ __device__ __forceinline__ void tile_pairs(const double pi[6], uint32_t i,
             }
         }
     } else {
-        constexpr uint32_t FOLD = NQ < 32 ? NQ : 32;
+        constexpr uint32_t FOLD = NQ < H4_MIXED_FOLD ? NQ : H4_MIXED_FOLD;
         const float e2f = (float)e2;
         for (uint32_t q0 = 0; q0 < NQ; q0 += FOLD) {
             float t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
