@@ -30,7 +30,7 @@ EXPORTS = (
     "h4h_init_dt", "h4h_find_particles_to_move", "h4h_set_active",
     "h4h_get_active", "h4h_save_old_acc_jrk", "h4h_predicted_pos_vel",
     "h4h_update_acc_jrk", "h4h_correction_pos_vel",
-    "h4h_next_integration_time", "h4h_step", "h4h_get_energy",
+    "h4h_next_integration_time", "h4h_step", "h4h_run", "h4h_get_energy",
     "h4h_set_timing", "h4h_get_stats", "h4h_synchronize",
 )
 
@@ -44,6 +44,23 @@ class H4HStats(ctypes.Structure):
         ("update_ms", ctypes.c_double),
         ("step_ms", ctypes.c_double),
     ]
+
+
+class H4HRunInfo(ctypes.Structure):
+    _fields_ = [
+        ("itime", ctypes.c_double),
+        ("atime", ctypes.c_double),
+        ("steps", ctypes.c_uint64),
+        ("interactions", ctypes.c_uint64),
+        ("nsteps", ctypes.c_uint64),
+        ("last_nact", ctypes.c_uint32),
+        ("stopped", ctypes.c_int),
+    ]
+
+
+H4H_STOP_OUTPUT = 1
+H4H_STOP_END = 2
+H4H_STOP_MAXSTEPS = 3
 
 
 class H4HError(RuntimeError):
@@ -93,6 +110,7 @@ def load():
         "h4h_correction_pos_vel": ([P, dbl, u32], i32),
         "h4h_next_integration_time": ([P, pd], i32),
         "h4h_step": ([P, dbl, pu, pd], i32),
+        "h4h_run": ([P, dbl, dbl, ctypes.c_uint64, ctypes.POINTER(H4HRunInfo), pd, pu], i32),
         "h4h_get_energy": ([P, pd, pd], i32),
         "h4h_set_timing": ([P, i32], i32),
         "h4h_get_stats": ([P, ctypes.POINTER(H4HStats), i32], i32),

@@ -38,11 +38,21 @@ struct Soa {
     double *pm;      // predictor mass, float-rounded (Predictor::m)
 };
 
-// Device-side scalars of one block step.
+// Device-side scalars of the block-step loop.  Every step kernel reads the
+// block time and the active count from here, so a step (or a captured
+// chain of steps) needs no host round trip.
 struct DevInfo {
-    uint32_t nact;                // active count (atomic compaction target)
-    uint32_t pad0;
-    unsigned long long tmin_key;  // order-preserving key of min(t+dt)
+    uint32_t nact;                    // active count (compaction target)
+    uint32_t done;                    // 1: the step loop has stopped
+    unsigned long long tmin_key;      // order-preserving key of min(t+dt)
+    double itime;                     // ITIME: block time of the current step
+    double atime;                     // ATIME: next block time
+    double t_end;                     // steps run while itime < t_end
+    double t_stop;                    // stop after a step with itime >= t_stop
+    unsigned long long steps;         // block steps since the last setup
+    unsigned long long max_steps;     // stop after this many steps
+    unsigned long long interactions;  // sum of nact * n
+    unsigned long long nsteps;        // sum of nact
 };
 
 // Order-preserving map double -> uint64 for atomicMin.

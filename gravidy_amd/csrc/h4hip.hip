@@ -99,12 +99,73 @@ __global__ __launch_bounds__(BS) void k_init_predictor(Soa s, uint32_t n,
     }
 }
 
-__global__ void k_begin_step(DevInfo *info) {
+// ---------------------------------------------------------------------------
+// Step-loop control (one thread)
+// ---------------------------------------------------------------------------
+
+// Arms the device step loop (h4h_run / h4h_step).
+__global__ void k_run_setup(DevInfo *info, double itime, double atime, double t_end,
+                            double t_stop, unsigned long long max_steps) {
+    info->done = 0;
+    info->itime = itime;
+    info->atime = atime;
+    info->t_end = t_end;
+    info->t_stop = t_stop;
+    info->steps = 0;
+    info->max_steps = max_steps;
+    info->interactions = 0;
+    info->nsteps = 0;
+}
+
+// Loop head of Hermite4CPU::integration (Hermite4CPU_integration.cpp:81-83):
+// stop unless ITIME < t_end, then ITIME = ATIME; reset the active count
+// and the next-time minimum.
+__global__ void k_step_begin(DevInfo *info) {
+    if (info->done) return;
+    if (!(info->itime < info->t_end)) {
+        info->done = 1;
+        return;
+    }
+    info->itime = info->atime;
     info->nact = 0;
     info->tmin_key = dkey(INFINITY);
 }
 
-__global__ void k_set_nact(DevInfo *info, uint32_t nact) { info->nact = nact; }
+// Loop tail: ATIME = min(t+dt), counters, per-step history, and the stop
+// test of the output interval (Hermite4CPU_integration.cpp:96-102).
+__global__ void k_step_end(DevInfo *info, uint32_t n, double *hist_t, uint32_t *hist_n,
+                           uint32_t hist_cap) {
+    if (info->done) return;
+    info->atime = dkey_inv(info->tmin_key);
+    unsigned long long k = info->steps;
+    hist_t[k % hist_cap] = info->itime;
+    hist_n[k % hist_cap] = info->nact;
+    info->steps = k + 1;
+    info->interactions += (unsigned long long)info->nact * n;
+    info->nsteps += info->nact;
+    if (info->itime >= info->t_stop || info->steps >= info->max_steps) info->done = 1;
+}
+
+// Single-method ABI calls: set the block time / reset counters.
+__global__ void k_set_time(DevInfo *info, double itime) {
+    info->done = 0;
+    info->itime = itime;
+}
+
+__global__ void k_begin_select(DevInfo *info) {
+    info->done = 0;
+    info->nact = 0;
+}
+
+__global__ void k_reset_tmin(DevInfo *info) {
+    info->done = 0;
+    info->tmin_key = dkey(INFINITY);
+}
+
+__global__ void k_set_nact(DevInfo *info, uint32_t nact) {
+    info->done = 0;
+    info->nact = nact;
+}
 
 __global__ __launch_bounds__(BS) void k_identity_active(uint32_t *move,
                                                        uint32_t n) {
@@ -120,14 +181,16 @@ enum : uint32_t { PS_PREDICT = 1, PS_SELECT = 2, PS_SAVE_OLD = 4 };
 
 // PS_PREDICT: Hermite4CPU::predicted_pos_vel (Hermite4CPU.cpp:340-365).
 // PS_SELECT:  Hermite4::find_particles_to_move (Hermite4.cpp:63-84): the
-//             active set is appended to move[] with one atomic per wave.
+//             active set is appended to move[] with one atomic per
+//             workgroup (ballot per wave, prefix over the waves in LDS).
 // PS_SAVE_OLD: Hermite4::save_old_acc_jrk (Hermite4.cpp:145-153).
-__global__ __launch_bounds__(BS) void k_predict_select(Soa s, double itime,
-                                                      uint32_t n,
-                                                      uint32_t flags,
-                                                      uint32_t *move,
-                                                      DevInfo *info) {
+// The block time is info->itime.
+__global__ __launch_bounds__(BS) void k_predict_select(Soa s, uint32_t n, uint32_t flags,
+                                                      uint32_t *move, DevInfo *info) {
 #pragma clang fp contract(off)
+    __shared__ uint32_t wave_off[BS / 64 + 1];
+    if (info->done) return;
+    const double itime = info->itime;
     uint32_t i = blockIdx.x * BS + threadIdx.x;
     bool in = i < n;
     bool act = false;
@@ -158,16 +221,23 @@ __global__ __launch_bounds__(BS) void k_predict_select(Soa s, double itime,
     }
     if (flags & PS_SELECT) {
         unsigned long long mask = __ballot(act);
-        uint32_t cnt = (uint32_t)__popcll(mask);
         uint32_t lane = __lane_id();
-        uint32_t base = 0;
-        if (cnt) {
-            if (lane == 0) base = atomicAdd(&info->nact, cnt);
-            base = __shfl(base, 0);
+        uint32_t w = threadIdx.x >> 6;
+        if (lane == 0) wave_off[w] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t total = 0;
+            for (uint32_t k = 0; k < BS / 64; ++k) {
+                uint32_t c = wave_off[k];
+                wave_off[k] = total;
+                total += c;
+            }
+            wave_off[BS / 64] = total ? atomicAdd(&info->nact, total) : 0;
         }
+        __syncthreads();
         if (act) {
             unsigned long long below = mask & ((1ull << lane) - 1ull);
-            move[base + (uint32_t)__popcll(below)] = i;
+            move[wave_off[BS / 64] + wave_off[w] + (uint32_t)__popcll(below)] = i;
             if (flags & PS_SAVE_OLD)
                 for (int c = 0; c < 6; ++c) s.old[c][i] = s.f[c][i];
         }
@@ -215,6 +285,7 @@ __global__ __launch_bounds__(BS) void k_update_exact(Soa s,
                                                     uint32_t n, double e2) {
     __shared__ double4 sp[JT];
     __shared__ double4 sv[JT];
+    if (info->done) return;
     const uint32_t nact = info->nact;
     const uint32_t n_it = (nact + BS - 1) / BS;
     for (uint32_t it = blockIdx.x; it < n_it; it += gridDim.x) {
@@ -333,17 +404,10 @@ __device__ __forceinline__ void reduce_subgroups(double fo[6], double *red) {
 // partial sums go to partial[c][chunk][k] and k_reduce_partials adds them
 // in chunk order.
 template <int IW, bool MIXED>
-__global__ __launch_bounds__(BS) void k_update_fp64(Soa s,
-                                                   const uint32_t *move,
-                                                   const DevInfo *info,
-                                                   double *partial,
-                                                   uint32_t n_jt,
-                                                   uint32_t max_split,
-                                                   double e2) {
-    __shared__ double4 sp[JT];
-    __shared__ double4 sv[JT];
-    __shared__ double red[(IW < BS ? BS : 1) * 6];
-    const uint32_t nact = info->nact;
+__device__ __forceinline__ void update_body(const Soa &s, const uint32_t *move, uint32_t nact,
+                                            double *partial, uint32_t n_jt,
+                                            uint32_t max_split, double e2, double4 *sp,
+                                            double4 *sv, double *red) {
     const Decomp d = decompose(nact, IW, n_jt, gridDim.x, max_split);
     const uint32_t il = threadIdx.x % IW;
     const uint32_t js = threadIdx.x / IW;
@@ -382,6 +446,36 @@ __global__ __launch_bounds__(BS) void k_update_fp64(Soa s,
     }
 }
 
+// One kernel for every active-set size: the lane-group width IW is picked
+// on the device from nact (pick_iw), so the launch needs no host knowledge
+// of nact.
+template <bool MIXED>
+__global__ __launch_bounds__(BS) void k_update_fp64(Soa s, const uint32_t *move,
+                                                   const DevInfo *info, double *partial,
+                                                   uint32_t n_jt, uint32_t max_split,
+                                                   double e2) {
+    __shared__ double4 sp[JT];
+    __shared__ double4 sv[JT];
+    __shared__ double red[BS * 6];
+    if (info->done) return;
+    const uint32_t nact = info->nact;
+    if (nact == 0) return;
+    switch (pick_iw(nact)) {
+        case 4:
+            update_body<4, MIXED>(s, move, nact, partial, n_jt, max_split, e2, sp, sv, red);
+            break;
+        case 16:
+            update_body<16, MIXED>(s, move, nact, partial, n_jt, max_split, e2, sp, sv, red);
+            break;
+        case 64:
+            update_body<64, MIXED>(s, move, nact, partial, n_jt, max_split, e2, sp, sv, red);
+            break;
+        default:
+            update_body<256, MIXED>(s, move, nact, partial, n_jt, max_split, e2, sp, sv, red);
+            break;
+    }
+}
+
 // Adds the S chunk partials of every active particle in chunk order.
 // Few chunks: one lane per particle.  Many chunks: one workgroup per
 // particle, lanes striding over chunks, then a fixed-order tree.
@@ -389,13 +483,14 @@ __global__ __launch_bounds__(BS) void k_reduce_partials(Soa s,
                                                        const uint32_t *move,
                                                        const DevInfo *info,
                                                        const double *partial,
-                                                       uint32_t iw,
                                                        uint32_t n_jt,
                                                        uint32_t grid,
                                                        uint32_t max_split) {
     __shared__ double red[BS / 64];
+    if (info->done) return;
     const uint32_t nact = info->nact;
-    const Decomp d = decompose(nact, iw, n_jt, grid, max_split);
+    if (nact == 0) return;
+    const Decomp d = decompose(nact, pick_iw(nact), n_jt, grid, max_split);
     if (d.S == 1) return;
     size_t plane = (size_t)d.S * d.rows;
     if (d.S <= 32) {
@@ -470,10 +565,11 @@ __device__ __forceinline__ double normalize_dt(double new_dt, double old_dt,
 // Hermite4CPU::correction_pos_vel (Hermite4CPU.cpp:370-411) with
 // NbodyUtils::get_timestep_normal (NbodyUtils.cpp:337-365).
 __global__ __launch_bounds__(BS) void k_correct(Soa s, const uint32_t *move,
-                                               const DevInfo *info,
-                                               double itime, float eta,
+                                               const DevInfo *info, float eta,
                                                double dt_min, double dt_max) {
 #pragma clang fp contract(off)
+    if (info->done) return;
+    const double itime = info->itime;
     const uint32_t nact = info->nact;
     for (uint32_t k = blockIdx.x * BS + threadIdx.x; k < nact;
          k += gridDim.x * BS) {
@@ -535,6 +631,7 @@ __global__ __launch_bounds__(BS) void k_next_time(const double *t,
                                                  const double *dt, uint32_t n,
                                                  DevInfo *info) {
     __shared__ double red[BS / 64];
+    if (info->done) return;
     double mn = INFINITY;
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < n;
          i += gridDim.x * BS)
@@ -680,6 +777,12 @@ __global__ __launch_bounds__(BS) void k_energy(Soa s, uint32_t n,
 // Context
 // ---------------------------------------------------------------------------
 
+namespace {
+constexpr uint32_t HIST_CAP = 64;      // per-step history ring (>= RUN_CHUNK)
+constexpr uint32_t RUN_CHUNK = 32;     // block steps per captured graph
+constexpr int N_SOA = 35;              // doubles per particle in `state`
+}  // namespace
+
 struct h4h_ctx {
     uint32_t n = 0;
     uint32_t n_pad = 0;
@@ -689,7 +792,8 @@ struct h4h_ctx {
     double dt_min = 0.0, dt_max = 0.0;
     int precision = H4H_FP64;
     int device = 0;
-    uint32_t grid = DEFAULT_GRID;
+    uint32_t grid = DEFAULT_GRID;     // requested persistent workgroups
+    uint32_t grid_eff = DEFAULT_GRID; // used (<= grid, bounded by the work n allows)
     uint32_t max_split = DEFAULT_MAX_SPLIT;
     bool uploaded = false;
     bool timing = false;
@@ -704,14 +808,18 @@ struct h4h_ctx {
     uint32_t *move = nullptr;
     DevInfo *info = nullptr;
     DevInfo *info_host = nullptr;  // pinned
+    double *hist_t = nullptr;
+    uint32_t *hist_n = nullptr;
+    double *hist_t_host = nullptr;   // pinned
+    uint32_t *hist_n_host = nullptr; // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     Soa s{};
     h4h_stats stats{};
 };
 
 namespace {
-
-constexpr int N_SOA = 35;  // doubles per particle in `state`
 
 int check_ctx(h4h_ctx *ctx, bool need_upload = true) {
     if (!ctx) return fail(H4H_EINVAL, "null context");
@@ -722,8 +830,31 @@ int check_ctx(h4h_ctx *ctx, bool need_upload = true) {
 
 uint32_t blocks_for(uint32_t n) { return (n + BS - 1) / BS; }
 
+// Largest number of update work items any active-set size can produce for
+// this n (the persistent grid is capped by it, so small systems do not
+// launch thousands of idle workgroups).
+uint32_t max_items(uint32_t n, uint32_t n_jt, uint32_t grid, uint32_t max_split) {
+    uint32_t best = 1;
+    for (uint32_t nact = 1;; nact = (nact < 64 ? nact + 1 : nact * 2)) {
+        uint32_t k = nact > n ? n : nact;
+        Decomp d = decompose(k, pick_iw(k), n_jt, grid, max_split);
+        best = std::max(best, d.n_items);
+        if (nact >= n) break;
+    }
+    return best;
+}
+
+void drop_graph(h4h_ctx *ctx) {
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+}
+
 int ensure_partial(h4h_ctx *ctx) {
     size_t need = (size_t)6 * 2 * ctx->grid * BS;
+    ctx->grid_eff = std::min(ctx->grid, max_items(ctx->n, ctx->n_jt, ctx->grid, ctx->max_split));
+    drop_graph(ctx);
     if (ctx->partial && ctx->partial_len >= need) return H4H_OK;
     if (ctx->partial) H4H_HIP(hipFree(ctx->partial));
     ctx->partial = nullptr;
@@ -732,57 +863,134 @@ int ensure_partial(h4h_ctx *ctx) {
     return H4H_OK;
 }
 
-}  // namespace
-
-// One update pass over the current active list.  The kernels read nact
-// from the device; k_update_fp64 is always launched with ctx->grid
-// workgroups because decompose() on the device is given gridDim.x and
-// k_reduce_partials must see the same value (extra workgroups exit).
-static int update_pass(h4h_ctx *ctx) {
-    if (ctx->nact_host == 0) return H4H_OK;
-    if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+// The update pass: pair kernel + chunk reduction.  Both read nact and the
+// stop flag from the device.
+int launch_update(h4h_ctx *ctx) {
     if (ctx->precision == H4H_FP64_EXACT) {
-        uint32_t grid = std::max<uint32_t>(1, std::min(ctx->grid, blocks_for(ctx->nact_host)));
-        hipLaunchKernelGGL(k_update_exact, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s,
+        hipLaunchKernelGGL(k_update_exact, dim3(ctx->grid_eff), dim3(BS), 0, ctx->stream, ctx->s,
                            ctx->move, ctx->info, ctx->n, ctx->e2);
         H4H_LAUNCHED();
-    } else {
-        uint32_t iw = pick_iw(ctx->nact_host);
-        bool mixed = ctx->precision == H4H_MIXED;
-        auto *kern = mixed ? k_update_fp64<256, true> : k_update_fp64<256, false>;
-        if (iw == 64) kern = mixed ? k_update_fp64<64, true> : k_update_fp64<64, false>;
-        else if (iw == 16) kern = mixed ? k_update_fp64<16, true> : k_update_fp64<16, false>;
-        else if (iw == 4) kern = mixed ? k_update_fp64<4, true> : k_update_fp64<4, false>;
-        hipLaunchKernelGGL(kern, dim3(ctx->grid), dim3(BS), 0, ctx->stream, ctx->s,
-                           ctx->move, ctx->info, ctx->partial, ctx->n_jt, ctx->max_split,
-                           ctx->e2);
-        H4H_LAUNCHED();
-        Decomp d = decompose(ctx->nact_host, iw, ctx->n_jt, ctx->grid, ctx->max_split);
-        if (d.S > 1) {
-            uint32_t rgrid = d.S <= 32 ? std::max<uint32_t>(1, std::min<uint32_t>(1024, blocks_for(ctx->nact_host)))
-                                       : std::min<uint32_t>(ctx->nact_host, 2048);
-            hipLaunchKernelGGL(k_reduce_partials, dim3(rgrid), dim3(BS), 0, ctx->stream, ctx->s,
-                               ctx->move, ctx->info, ctx->partial, iw, ctx->n_jt, ctx->grid,
-                               ctx->max_split);
-            H4H_LAUNCHED();
-        }
+        return H4H_OK;
     }
-    if (ctx->timing) {
-        H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-        H4H_HIP(hipEventSynchronize(ctx->ev[1]));
-        float ms = 0.f;
-        H4H_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-        ctx->stats.update_ms += ms;
-        ctx->stats.update_launches += 1;
-    }
-    ctx->stats.interactions += (uint64_t)ctx->nact_host * ctx->n;
-    ctx->stats.nsteps += ctx->nact_host;
+    auto *kern = ctx->precision == H4H_MIXED ? k_update_fp64<true> : k_update_fp64<false>;
+    hipLaunchKernelGGL(kern, dim3(ctx->grid_eff), dim3(BS), 0, ctx->stream, ctx->s, ctx->move,
+                       ctx->info, ctx->partial, ctx->n_jt, ctx->max_split, ctx->e2);
+    H4H_LAUNCHED();
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1024), dim3(BS), 0, ctx->stream, ctx->s, ctx->move,
+                       ctx->info, ctx->partial, ctx->n_jt, ctx->grid_eff, ctx->max_split);
+    H4H_LAUNCHED();
     return H4H_OK;
 }
 
+int launch_predict_select(h4h_ctx *ctx, uint32_t flags) {
+    hipLaunchKernelGGL(k_predict_select, dim3(blocks_for(ctx->n)), dim3(BS), 0, ctx->stream,
+                       ctx->s, ctx->n, flags, ctx->move, ctx->info);
+    H4H_LAUNCHED();
+    return H4H_OK;
+}
+
+int launch_correct(h4h_ctx *ctx) {
+    uint32_t grid = std::min<uint32_t>(blocks_for(ctx->n), 1024);
+    hipLaunchKernelGGL(k_correct, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s, ctx->move,
+                       ctx->info, ctx->eta, ctx->dt_min, ctx->dt_max);
+    H4H_LAUNCHED();
+    return H4H_OK;
+}
+
+int launch_next_time(h4h_ctx *ctx) {
+    uint32_t grid = std::min<uint32_t>(blocks_for(ctx->n), 512);
+    hipLaunchKernelGGL(k_next_time, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s.t, ctx->s.dt,
+                       ctx->n, ctx->info);
+    H4H_LAUNCHED();
+    return H4H_OK;
+}
+
+// One block step on the stream, entirely device-driven: loop head,
+// predictor + selection + save-old, update, corrector, next time, loop tail.
+int enqueue_step(h4h_ctx *ctx) {
+    int rc;
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
+    H4H_LAUNCHED();
+    if ((rc = launch_predict_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD))) return rc;
+    if ((rc = launch_update(ctx))) return rc;
+    if ((rc = launch_correct(ctx))) return rc;
+    if ((rc = launch_next_time(ctx))) return rc;
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, ctx->stream, ctx->info, ctx->n,
+                       ctx->hist_t, ctx->hist_n, HIST_CAP);
+    H4H_LAUNCHED();
+    return H4H_OK;
+}
+
+// Captures RUN_CHUNK steps into one graph (once per context and
+// configuration).  The graph holds kernel nodes only.
+int ensure_graph(h4h_ctx *ctx) {
+    if (ctx->graph_exec) return H4H_OK;
+    H4H_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = H4H_OK;
+    for (uint32_t k = 0; k < RUN_CHUNK && rc == H4H_OK; ++k) rc = enqueue_step(ctx);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(H4H_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    ctx->graph = g;
+    H4H_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+    return H4H_OK;
+}
+
+int read_info(h4h_ctx *ctx) {
+    H4H_HIP(hipMemcpyAsync(ctx->info_host, ctx->info, sizeof(DevInfo), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    return H4H_OK;
+}
+
+// The update pass with optional HIP-event timing (bench and stats).
+int update_pass(h4h_ctx *ctx, bool sync_for_timing) {
+    if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    int rc = launch_update(ctx);
+    if (rc) return rc;
+    if (ctx->timing) {
+        H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+        if (sync_for_timing) {
+            H4H_HIP(hipEventSynchronize(ctx->ev[1]));
+            float ms = 0.f;
+            H4H_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+            ctx->stats.update_ms += ms;
+            ctx->stats.update_launches += 1;
+        }
+    }
+    return H4H_OK;
+}
+
+int upload_cols(h4h_ctx *ctx, const double *src, uint32_t ncols, Cols dst) {
+    H4H_HIP(hipMemcpyAsync(ctx->staging, src, (size_t)ctx->n * ncols * sizeof(double),
+                           hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(blocks_for(ctx->n)), dim3(BS), 0, ctx->stream,
+                       ctx->staging, ncols, dst, ctx->n);
+    H4H_LAUNCHED();
+    return H4H_OK;
+}
+
+int download_cols(h4h_ctx *ctx, double *dst, uint32_t ncols, Cols src) {
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(blocks_for(ctx->n)), dim3(BS), 0, ctx->stream, src,
+                       ncols, ctx->staging, ctx->n);
+    H4H_LAUNCHED();
+    H4H_HIP(hipMemcpyAsync(dst, ctx->staging, (size_t)ctx->n * ncols * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    return H4H_OK;
+}
+
+bool valid_precision(int p) { return p == H4H_FP64 || p == H4H_FP64_EXACT || p == H4H_MIXED; }
+
+}  // namespace
+
 extern "C" {
 
-int h4h_abi_version(void) { return 100; }
+int h4h_abi_version(void) { return 101; }
 
 const char *h4h_last_error(void) { return g_err.c_str(); }
 
@@ -796,8 +1004,7 @@ int h4h_create(h4h_ctx **out, uint32_t n, double e2, float eta, double dt_min,
     if (!(eta > 0.0f) || !std::isfinite(eta)) return fail(H4H_EINVAL, "eta must be > 0");
     if (!(dt_min > 0.0) || !(dt_max >= dt_min) || !std::isfinite(dt_max))
         return fail(H4H_EINVAL, "need 0 < dt_min <= dt_max");
-    if (precision != H4H_FP64 && precision != H4H_FP64_EXACT && precision != H4H_MIXED)
-        return fail(H4H_EINVAL, "unknown precision");
+    if (!valid_precision(precision)) return fail(H4H_EINVAL, "unknown precision");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(H4H_ENODEV, "no HIP device available");
@@ -814,7 +1021,9 @@ int h4h_create(h4h_ctx **out, uint32_t n, double e2, float eta, double dt_min,
     ctx->precision = precision;
 
     auto cleanup = [&](int rc) {
+        std::string keep = g_err;
         h4h_destroy(ctx);
+        g_err = keep;
         return rc;
     };
     if (hipGetDevice(&ctx->device) != hipSuccess) return cleanup(fail(H4H_EHIP, "hipGetDevice failed"));
@@ -825,13 +1034,15 @@ int h4h_create(h4h_ctx **out, uint32_t n, double e2, float eta, double dt_min,
         return cleanup(fail(H4H_ENOMEM, "device allocation of the state failed"));
     if (hipMalloc(&ctx->staging, np * 8 * sizeof(double)) != hipSuccess)
         return cleanup(fail(H4H_ENOMEM, "device allocation of staging failed"));
-    if (hipMalloc(&ctx->eout, ENERGY_GRID * 4 * sizeof(double)) != hipSuccess)
+    if (hipMalloc(&ctx->eout, ENERGY_GRID * 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->move, np * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&ctx->info, sizeof(DevInfo)) != hipSuccess ||
+        hipMalloc(&ctx->hist_t, HIST_CAP * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->hist_n, HIST_CAP * sizeof(uint32_t)) != hipSuccess)
         return cleanup(fail(H4H_ENOMEM, "device allocation failed"));
-    if (hipMalloc(&ctx->move, np * sizeof(uint32_t)) != hipSuccess)
-        return cleanup(fail(H4H_ENOMEM, "device allocation failed"));
-    if (hipMalloc(&ctx->info, sizeof(DevInfo)) != hipSuccess)
-        return cleanup(fail(H4H_ENOMEM, "device allocation failed"));
-    if (hipHostMalloc(&ctx->info_host, sizeof(DevInfo), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc(&ctx->info_host, sizeof(DevInfo), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&ctx->hist_t_host, HIST_CAP * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&ctx->hist_n_host, HIST_CAP * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(H4H_ENOMEM, "pinned allocation failed"));
     for (int k = 0; k < 4; ++k)
         if (hipEventCreate(&ctx->ev[k]) != hipSuccess)
@@ -857,9 +1068,9 @@ int h4h_create(h4h_ctx **out, uint32_t n, double e2, float eta, double dt_min,
     s.dt = take();
     for (int c = 0; c < 6; ++c) s.p[c] = take();
     s.pm = take();
-    if (hipMemsetAsync(ctx->state, 0, np * N_SOA * sizeof(double), ctx->stream) != hipSuccess)
+    if (hipMemsetAsync(ctx->state, 0, np * N_SOA * sizeof(double), ctx->stream) != hipSuccess ||
+        hipMemsetAsync(ctx->info, 0, sizeof(DevInfo), ctx->stream) != hipSuccess)
         return cleanup(fail(H4H_EHIP, "hipMemset failed"));
-    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess)
         return cleanup(fail(H4H_EHIP, "initialisation failed"));
     *out = ctx;
@@ -869,6 +1080,7 @@ int h4h_create(h4h_ctx **out, uint32_t n, double e2, float eta, double dt_min,
 int h4h_destroy(h4h_ctx *ctx) {
     if (!ctx) return H4H_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drop_graph(ctx);
     for (int k = 0; k < 4; ++k)
         if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
     if (ctx->state) (void)hipFree(ctx->state);
@@ -877,7 +1089,11 @@ int h4h_destroy(h4h_ctx *ctx) {
     if (ctx->eout) (void)hipFree(ctx->eout);
     if (ctx->move) (void)hipFree(ctx->move);
     if (ctx->info) (void)hipFree(ctx->info);
+    if (ctx->hist_t) (void)hipFree(ctx->hist_t);
+    if (ctx->hist_n) (void)hipFree(ctx->hist_n);
     if (ctx->info_host) (void)hipHostFree(ctx->info_host);
+    if (ctx->hist_t_host) (void)hipHostFree(ctx->hist_t_host);
+    if (ctx->hist_n_host) (void)hipHostFree(ctx->hist_n_host);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return H4H_OK;
@@ -886,8 +1102,8 @@ int h4h_destroy(h4h_ctx *ctx) {
 int h4h_set_precision(h4h_ctx *ctx, int precision) {
     int rc = check_ctx(ctx, false);
     if (rc) return rc;
-    if (precision != H4H_FP64 && precision != H4H_FP64_EXACT && precision != H4H_MIXED)
-        return fail(H4H_EINVAL, "unknown precision");
+    if (!valid_precision(precision)) return fail(H4H_EINVAL, "unknown precision");
+    if (precision != ctx->precision) drop_graph(ctx);
     ctx->precision = precision;
     return H4H_OK;
 }
@@ -899,25 +1115,6 @@ int h4h_set_launch(h4h_ctx *ctx, uint32_t workgroups, uint32_t max_jsplit) {
     ctx->grid = workgroups ? workgroups : DEFAULT_GRID;
     ctx->max_split = max_jsplit ? max_jsplit : DEFAULT_MAX_SPLIT;
     return ensure_partial(ctx);
-}
-
-static int upload_cols(h4h_ctx *ctx, const double *src, uint32_t ncols, Cols dst) {
-    H4H_HIP(hipMemcpyAsync(ctx->staging, src, (size_t)ctx->n * ncols * sizeof(double),
-                           hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_aos_to_soa, dim3(blocks_for(ctx->n)), dim3(BS), 0, ctx->stream,
-                       ctx->staging, ncols, dst, ctx->n);
-    H4H_LAUNCHED();
-    return H4H_OK;
-}
-
-static int download_cols(h4h_ctx *ctx, double *dst, uint32_t ncols, Cols src) {
-    hipLaunchKernelGGL(k_soa_to_aos, dim3(blocks_for(ctx->n)), dim3(BS), 0, ctx->stream, src,
-                       ncols, ctx->staging, ctx->n);
-    H4H_LAUNCHED();
-    H4H_HIP(hipMemcpyAsync(dst, ctx->staging, (size_t)ctx->n * ncols * sizeof(double),
-                           hipMemcpyDeviceToHost, ctx->stream));
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
-    return H4H_OK;
 }
 
 int h4h_upload(h4h_ctx *ctx, const double *r4, const double *v4, const double *f6,
@@ -945,7 +1142,7 @@ int h4h_upload(h4h_ctx *ctx, const double *r4, const double *v4, const double *f
     hipLaunchKernelGGL(k_init_predictor, dim3(blocks_for(ctx->n_pad)), dim3(BS), 0, ctx->stream,
                        ctx->s, ctx->n, ctx->n_pad);
     H4H_LAUNCHED();
-    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
+    hipLaunchKernelGGL(k_set_nact, dim3(1), dim3(1), 0, ctx->stream, ctx->info, 0u);
     H4H_LAUNCHED();
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     ctx->nact_host = 0;
@@ -1027,6 +1224,7 @@ int h4h_upload_predictor(h4h_ctx *ctx, const double *p6, const float *pm) {
         for (uint32_t i = 0; i < ctx->n; ++i) tmp[i] = (double)pm[i];
         H4H_HIP(hipMemcpyAsync(s.pm, tmp.data(), ctx->n * sizeof(double), hipMemcpyHostToDevice,
                                ctx->stream));
+        H4H_HIP(hipStreamSynchronize(ctx->stream));
     }
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     return H4H_OK;
@@ -1072,8 +1270,7 @@ int h4h_init_acc_jrk(h4h_ctx *ctx) {
     hipLaunchKernelGGL(k_set_nact, dim3(1), dim3(1), 0, ctx->stream, ctx->info, ctx->n);
     H4H_LAUNCHED();
     ctx->nact_host = ctx->n;
-    rc = update_pass(ctx);
-    if (rc) return rc;
+    if ((rc = update_pass(ctx, true))) return rc;
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     return H4H_OK;
 }
@@ -1082,29 +1279,15 @@ int h4h_init_dt(h4h_ctx *ctx, double *ctime, float eta, double itime) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!ctime) return fail(H4H_EINVAL, "ctime is null");
-    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
+    hipLaunchKernelGGL(k_reset_tmin, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
     H4H_LAUNCHED();
     uint32_t grid = std::min<uint32_t>(blocks_for(ctx->n), 1024);
     hipLaunchKernelGGL(k_init_dt, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s, ctx->n, eta, itime,
                        ctx->dt_min, ctx->dt_max, ctx->info);
     H4H_LAUNCHED();
-    H4H_HIP(hipMemcpyAsync(ctx->info_host, ctx->info, sizeof(DevInfo), hipMemcpyDeviceToHost,
-                           ctx->stream));
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    if ((rc = read_info(ctx))) return rc;
     double mn = dkey_inv(ctx->info_host->tmin_key);
     if (mn < *ctime) *ctime = mn;
-    ctx->nact_host = 0;
-    return H4H_OK;
-}
-
-static int predict_select(h4h_ctx *ctx, double itime, uint32_t flags) {
-    if (flags & PS_SELECT) {
-        hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
-        H4H_LAUNCHED();
-    }
-    hipLaunchKernelGGL(k_predict_select, dim3(blocks_for(ctx->n)), dim3(BS), 0, ctx->stream,
-                       ctx->s, itime, ctx->n, flags, ctx->move, ctx->info);
-    H4H_LAUNCHED();
     return H4H_OK;
 }
 
@@ -1112,10 +1295,12 @@ int h4h_find_particles_to_move(h4h_ctx *ctx, double itime, uint32_t *nact) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!nact) return fail(H4H_EINVAL, "nact is null");
-    if ((rc = predict_select(ctx, itime, PS_SELECT))) return rc;
-    H4H_HIP(hipMemcpyAsync(ctx->info_host, ctx->info, sizeof(DevInfo), hipMemcpyDeviceToHost,
-                           ctx->stream));
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, ctx->stream, ctx->info, itime);
+    H4H_LAUNCHED();
+    hipLaunchKernelGGL(k_begin_select, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
+    H4H_LAUNCHED();
+    if ((rc = launch_predict_select(ctx, PS_SELECT))) return rc;
+    if ((rc = read_info(ctx))) return rc;
     ctx->nact_host = ctx->info_host->nact;
     *nact = ctx->nact_host;
     return H4H_OK;
@@ -1136,7 +1321,9 @@ int h4h_save_old_acc_jrk(h4h_ctx *ctx) {
 int h4h_predicted_pos_vel(h4h_ctx *ctx, double itime) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
-    if ((rc = predict_select(ctx, itime, PS_PREDICT))) return rc;
+    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, ctx->stream, ctx->info, itime);
+    H4H_LAUNCHED();
+    if ((rc = launch_predict_select(ctx, PS_PREDICT))) return rc;
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     return H4H_OK;
 }
@@ -1146,18 +1333,13 @@ int h4h_update_acc_jrk(h4h_ctx *ctx, uint32_t nact) {
     if (rc) return rc;
     if (nact != ctx->nact_host)
         return fail(H4H_EINVAL, "nact differs from the current active list size");
-    if ((rc = update_pass(ctx))) return rc;
+    if (nact) {
+        if ((rc = update_pass(ctx, true))) return rc;
+    }
     ctx->stats.block_steps += 1;
+    ctx->stats.interactions += (uint64_t)nact * ctx->n;
+    ctx->stats.nsteps += nact;
     H4H_HIP(hipStreamSynchronize(ctx->stream));
-    return H4H_OK;
-}
-
-static int correct_pass(h4h_ctx *ctx, double itime) {
-    if (!ctx->nact_host) return H4H_OK;
-    uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(blocks_for(ctx->nact_host), 1024));
-    hipLaunchKernelGGL(k_correct, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s, ctx->move,
-                       ctx->info, itime, ctx->eta, ctx->dt_min, ctx->dt_max);
-    H4H_LAUNCHED();
     return H4H_OK;
 }
 
@@ -1166,16 +1348,10 @@ int h4h_correction_pos_vel(h4h_ctx *ctx, double itime, uint32_t nact) {
     if (rc) return rc;
     if (nact != ctx->nact_host)
         return fail(H4H_EINVAL, "nact differs from the current active list size");
-    if ((rc = correct_pass(ctx, itime))) return rc;
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
-    return H4H_OK;
-}
-
-static int next_time_pass(h4h_ctx *ctx) {
-    uint32_t grid = std::min<uint32_t>(blocks_for(ctx->n), 512);
-    hipLaunchKernelGGL(k_next_time, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s.t, ctx->s.dt,
-                       ctx->n, ctx->info);
+    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, ctx->stream, ctx->info, itime);
     H4H_LAUNCHED();
+    if ((rc = launch_correct(ctx))) return rc;
+    H4H_HIP(hipStreamSynchronize(ctx->stream));
     return H4H_OK;
 }
 
@@ -1183,16 +1359,10 @@ int h4h_next_integration_time(h4h_ctx *ctx, double *atime) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!atime) return fail(H4H_EINVAL, "atime is null");
-    // Reset only the min key: the active count stays valid.
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
-    DevInfo reset{};
-    reset.nact = ctx->nact_host;
-    reset.tmin_key = dkey(INFINITY);
-    H4H_HIP(hipMemcpyAsync(ctx->info, &reset, sizeof(DevInfo), hipMemcpyHostToDevice, ctx->stream));
-    if ((rc = next_time_pass(ctx))) return rc;
-    H4H_HIP(hipMemcpyAsync(ctx->info_host, ctx->info, sizeof(DevInfo), hipMemcpyDeviceToHost,
-                           ctx->stream));
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    hipLaunchKernelGGL(k_reset_tmin, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
+    H4H_LAUNCHED();
+    if ((rc = launch_next_time(ctx))) return rc;
+    if ((rc = read_info(ctx))) return rc;
     *atime = dkey_inv(ctx->info_host->tmin_key);
     return H4H_OK;
 }
@@ -1201,27 +1371,93 @@ int h4h_step(h4h_ctx *ctx, double itime, uint32_t *nact, double *atime) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-    if ((rc = predict_select(ctx, itime, PS_PREDICT | PS_SELECT | PS_SAVE_OLD))) return rc;
-    // The active count is needed on the host to size the launches.
-    H4H_HIP(hipMemcpyAsync(ctx->info_host, ctx->info, sizeof(DevInfo), hipMemcpyDeviceToHost,
-                           ctx->stream));
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->nact_host = ctx->info_host->nact;
-    if ((rc = update_pass(ctx))) return rc;
-    if ((rc = correct_pass(ctx, itime))) return rc;
-    if ((rc = next_time_pass(ctx))) return rc;
-    H4H_HIP(hipMemcpyAsync(ctx->info_host, ctx->info, sizeof(DevInfo), hipMemcpyDeviceToHost,
-                           ctx->stream));
+    // One-step loop: ITIME = ATIME = itime, stop after the step.
+    hipLaunchKernelGGL(k_run_setup, dim3(1), dim3(1), 0, ctx->stream, ctx->info, -INFINITY, itime,
+                       (double)INFINITY, -INFINITY, 1ull);
+    H4H_LAUNCHED();
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
+    H4H_LAUNCHED();
+    if ((rc = launch_predict_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD))) return rc;
+    if ((rc = update_pass(ctx, false))) return rc;
+    if ((rc = launch_correct(ctx))) return rc;
+    if ((rc = launch_next_time(ctx))) return rc;
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, ctx->stream, ctx->info, ctx->n,
+                       ctx->hist_t, ctx->hist_n, HIST_CAP);
+    H4H_LAUNCHED();
     if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    if ((rc = read_info(ctx))) return rc;
     if (ctx->timing) {
         float ms = 0.f;
         H4H_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
         ctx->stats.step_ms += ms;
+        H4H_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        ctx->stats.update_ms += ms;
+        ctx->stats.update_launches += 1;
     }
+    ctx->nact_host = ctx->info_host->nact;
     ctx->stats.block_steps += 1;
+    ctx->stats.interactions += (uint64_t)ctx->nact_host * ctx->n;
+    ctx->stats.nsteps += ctx->nact_host;
     if (nact) *nact = ctx->nact_host;
-    if (atime) *atime = dkey_inv(ctx->info_host->tmin_key);
+    if (atime) *atime = ctx->info_host->atime;
+    return H4H_OK;
+}
+
+int h4h_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_steps, h4h_run_info *io,
+            double *step_time, uint32_t *step_nact) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!io) return fail(H4H_EINVAL, "io is null");
+    if (max_steps == 0) return fail(H4H_EINVAL, "max_steps must be >= 1");
+    if ((rc = ensure_graph(ctx))) return rc;
+    hipLaunchKernelGGL(k_run_setup, dim3(1), dim3(1), 0, ctx->stream, ctx->info, io->itime,
+                       io->atime, t_end, t_stop, (unsigned long long)max_steps);
+    H4H_LAUNCHED();
+    uint64_t seen = 0;
+    if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+    for (;;) {
+        H4H_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+        H4H_HIP(hipMemcpyAsync(ctx->info_host, ctx->info, sizeof(DevInfo), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        if (step_time || step_nact) {
+            H4H_HIP(hipMemcpyAsync(ctx->hist_t_host, ctx->hist_t, HIST_CAP * sizeof(double),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            H4H_HIP(hipMemcpyAsync(ctx->hist_n_host, ctx->hist_n, HIST_CAP * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        }
+        H4H_HIP(hipStreamSynchronize(ctx->stream));
+        const DevInfo &d = *ctx->info_host;
+        for (uint64_t k = seen; k < d.steps; ++k) {
+            if (step_time) step_time[k] = ctx->hist_t_host[k % HIST_CAP];
+            if (step_nact) step_nact[k] = ctx->hist_n_host[k % HIST_CAP];
+        }
+        seen = d.steps;
+        if (d.done) break;
+    }
+    if (ctx->timing) {
+        H4H_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
+        H4H_HIP(hipEventSynchronize(ctx->ev[3]));
+        float ms = 0.f;
+        H4H_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+        ctx->stats.step_ms += ms;
+    }
+    const DevInfo &d = *ctx->info_host;
+    io->itime = d.itime;
+    io->atime = d.atime;
+    io->steps = d.steps;
+    io->interactions = d.interactions;
+    io->nsteps = d.nsteps;
+    io->last_nact = d.steps ? d.nact : 0;
+    if (d.steps && d.itime >= t_stop)
+        io->stopped = H4H_STOP_OUTPUT;
+    else if (d.steps >= max_steps)
+        io->stopped = H4H_STOP_MAXSTEPS;
+    else
+        io->stopped = H4H_STOP_END;
+    ctx->nact_host = io->last_nact;
+    ctx->stats.block_steps += d.steps;
+    ctx->stats.interactions += d.interactions;
+    ctx->stats.nsteps += d.nsteps;
     return H4H_OK;
 }
 

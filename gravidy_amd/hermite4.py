@@ -202,6 +202,21 @@ class Hermite4HIP:
                                             ctypes.byref(a)))
         return k.value, a.value
 
+    def run(self, itime, atime, t_end, t_stop, max_steps, record=True):
+        """Block steps on the device until a step with ITIME >= t_stop is
+        done, ITIME >= t_end at the loop test, or max_steps steps (h4h_run).
+        Returns (info dict, step times, step nacts)."""
+        io = _lib.H4HRunInfo(float(itime), float(atime), 0, 0, 0, 0, 0)
+        st = np.zeros(max_steps) if record else None
+        sn = np.zeros(max_steps, np.uint32) if record else None
+        check("h4h_run", self.lib.h4h_run(
+            self._ctx, float(t_end), float(t_stop), int(max_steps), ctypes.byref(io),
+            None if st is None else _dp(st),
+            None if sn is None else sn.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        info = {k: getattr(io, k) for k, _ in io._fields_}
+        k = info["steps"]
+        return info, (st[:k] if record else None), (sn[:k] if record else None)
+
     def get_energy(self):
         ek = ctypes.c_double()
         ep = ctypes.c_double()
@@ -213,18 +228,64 @@ class Hermite4HIP:
         check("h4h_synchronize", self.lib.h4h_synchronize(self._ctx))
 
     def integration(self, t_end, interval=0.125, eta_s=ETA_DEFAULT, max_steps=None,
-                    on_output=None):
+                    on_output=None, chunk_steps=65536, mode="run"):
         """Hermite4CPU::integration's loop (Hermite4CPU_integration.cpp:38-130)
-        with every kernel on the GPU.  integration_time and interval_time are
-        float as in the reference (NbodySystem.hpp:69-70) and the interval
-        test is evaluated in float.  Returns the schedule (ITIME and nact per
-        block step) and the energies at t=0 and at every output time."""
-        t_end32 = np.float32(t_end)
+        with every kernel on the GPU; the block steps between two output
+        times run on the device (h4h_run).  integration_time and
+        interval_time are float as in the reference (NbodySystem.hpp:69-70)
+        and the interval test ITIME >= interval * k is evaluated in float.
+        mode="step" drives the same loop one h4h_step call per block step.
+        Returns the schedule (ITIME and nact per block step) and the
+        energies at t=0 and at every output time."""
+        if mode == "step":
+            return self._integration_steps(t_end, interval, eta_s, max_steps, on_output)
+        t_end32 = float(np.float32(t_end))
         iv32 = np.float32(interval)
-        atime = 1.0e10
         itime = 0.0
         self.init_acc_jrk()
-        atime = self.init_dt(atime, eta_s, itime)
+        atime = self.init_dt(1.0e10, eta_s, itime)
+        ek, ep = self.get_energy()
+        e_time, e_val = [itime], [ek + ep]
+        if on_output:
+            on_output(self, itime)
+        steps_t, steps_n = [], []
+        output_factor = 1
+        interactions = 0
+        done = 0
+        while itime < t_end32:
+            budget = chunk_steps if max_steps is None else min(chunk_steps, max_steps - done)
+            if budget <= 0:
+                break
+            t_stop = float(np.float32(iv32 * np.float32(output_factor)))
+            info, st, sn = self.run(itime, atime, t_end32, t_stop, budget)
+            steps_t.append(st)
+            steps_n.append(sn)
+            done += info["steps"]
+            interactions += info["interactions"]
+            if info["steps"]:
+                itime, atime = info["itime"], info["atime"]
+            if info["stopped"] == _lib.H4H_STOP_OUTPUT:
+                ek, ep = self.get_energy()
+                e_time.append(itime)
+                e_val.append(ek + ep)
+                if on_output:
+                    on_output(self, itime)
+                output_factor += 1
+            elif info["stopped"] == _lib.H4H_STOP_END:
+                break
+        return {
+            "step_time": np.concatenate(steps_t) if steps_t else np.zeros(0),
+            "step_nact": (np.concatenate(steps_n) if steps_n else np.zeros(0)).astype(np.uint32),
+            "energy_time": np.array(e_time), "energy": np.array(e_val),
+            "interactions": interactions,
+        }
+
+    def _integration_steps(self, t_end, interval, eta_s, max_steps, on_output):
+        t_end32 = np.float32(t_end)
+        iv32 = np.float32(interval)
+        itime = 0.0
+        self.init_acc_jrk()
+        atime = self.init_dt(1.0e10, eta_s, itime)
         ek, ep = self.get_energy()
         e_time, e_val = [itime], [ek + ep]
         if on_output:

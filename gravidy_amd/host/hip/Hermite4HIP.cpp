@@ -86,8 +86,8 @@ unsigned int Hermite4HIP::step(double ITIME, double &ATIME) {
 }
 
 // Same order of operations and output points as the reference loop; the
-// four per-step methods are one fused device step, and the host arrays
-// are refreshed from the device only where the output layer reads them.
+// per-step methods are one fused device step replayed on the device, and
+// the host arrays are refreshed only where the output layer reads them.
 // The interaction counter is 64-bit (the reference multiplies two 32-bit
 // values, Hermite4CPU_integration.cpp:96).  nbody_attributes() is not
 // called: its results are written nowhere.
@@ -122,12 +122,27 @@ void Hermite4HIP::integration() {
         logger->print_lagrange_radii(ITIME, nu->layers_radii);
     }
 
+    // The block steps between two output times run on the device
+    // (h4h_run); it stops after the step whose ITIME passes the output
+    // threshold, where the reference's loop prints.  Iter and Nsteps are
+    // printed before that step is counted, as in the reference.
     while (ITIME < ns->integration_time) {
-        ITIME = ATIME;
-        nact = step(ITIME, ATIME);
-        interactions += (long long)nact * (long long)ns->n;
+        h4h_run_info io;
+        io.itime = ITIME;
+        io.atime = ATIME;
+        double t_stop = ns->interval_time * output_factor;
+        ns->gtime.update_ini = omp_get_wtime();
+        check(h4h_run(ctx, ns->integration_time, t_stop, ~0ull, &io, NULL, NULL), "h4h_run");
+        ns->gtime.update_end += omp_get_wtime() - ns->gtime.update_ini;
+        if (io.steps == 0) break;
+        ITIME = io.itime;
+        ATIME = io.atime;
+        nact = io.last_nact;
+        interactions += (long long)io.interactions;
+        ns->iterations += (long long)io.steps - 1;
+        nsteps += (unsigned int)(io.nsteps - io.last_nact);
 
-        if (ITIME >= ns->interval_time * output_factor) {
+        if (io.stopped == H4H_STOP_OUTPUT) {
             double energy = get_energy_gpu();
             logger->print_energy_log(ITIME, ns->iterations, interactions, nsteps, energy);
             download();

@@ -175,6 +175,34 @@ int h4h_next_integration_time(h4h_ctx *ctx, double *atime);
  * (cpu/Hermite4CPU_integration.cpp:83-99). */
 int h4h_step(h4h_ctx *ctx, double itime, uint32_t *nact, double *atime);
 
+/* Result of h4h_run. */
+typedef struct h4h_run_info {
+    double   itime;         /* in: current block time (ITIME);
+                               out: block time of the last step done       */
+    double   atime;         /* in: next block time (ATIME); out: same      */
+    uint64_t steps;         /* out: block steps done                       */
+    uint64_t interactions;  /* out: sum of nact * n over those steps       */
+    uint64_t nsteps;        /* out: sum of nact over those steps           */
+    uint32_t last_nact;     /* out: nact of the last step                  */
+    int      stopped;       /* out: H4H_STOP_* (why the loop stopped)      */
+} h4h_run_info;
+
+#define H4H_STOP_OUTPUT   1  /* a step with itime >= t_stop was done      */
+#define H4H_STOP_END      2  /* itime >= t_end at the loop test           */
+#define H4H_STOP_MAXSTEPS 3  /* max_steps steps were done                 */
+
+/* Runs block steps on the device the way the reference's integration loop
+ * does (cpu/Hermite4CPU_integration.cpp:81-124):
+ *   while (ITIME < t_end) { ITIME = ATIME; step(ITIME); ... }
+ * starting from io->itime / io->atime, until a step whose ITIME >= t_stop
+ * has been done (the loop's output test, Hermite4CPU_integration.cpp:102,
+ * with t_stop = interval_time * output_factor), or ITIME >= t_end, or
+ * max_steps steps.  Steps are replayed as captured HIP graphs with no host
+ * round trip per step.  step_time / step_nact (each max_steps long, or
+ * NULL) receive ITIME and nact of every step. */
+int h4h_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_steps,
+            h4h_run_info *io, double *step_time, uint32_t *step_nact);
+
 /* Kinetic and (unsoftened) potential energy of the corrected state.
  * Replaces: NbodyUtils::get_energy (utils/NbodyUtils.cpp:427-529) and
  * Hermite4GPU::get_energy_gpu + k_energy (gpu/Hermite4GPU.cu:540-592,

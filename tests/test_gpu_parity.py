@@ -285,6 +285,32 @@ def test_c1_run_exact_is_bitwise(oracle):
     np.testing.assert_allclose(run["energy"], ref["energy"], rtol=1e-13)
 
 
+def test_device_loop_matches_host_loop(oracle):
+    """h4h_run (graph-replayed device loop) and one h4h_step call per block
+    step give the same schedule and bit-identical state, and the device
+    loop honours max_steps and the output threshold."""
+    r, v = ic.plummer(2048, 77)
+    out = {}
+    for mode in ("run", "step"):
+        with Hermite4HIP(r, v, E2, precision="fp64") as h:
+            res = h.integration(0.25, mode=mode)
+            out[mode] = (res, h.download())
+    (ra, sa), (rb, sb) = out["run"], out["step"]
+    assert np.array_equal(ra["step_time"], rb["step_time"])
+    assert np.array_equal(ra["step_nact"], rb["step_nact"])
+    assert bits_equal(sa["r"], sb["r"]) and bits_equal(sa["dt"], sb["dt"])
+    np.testing.assert_array_equal(ra["energy"], rb["energy"])
+    with Hermite4HIP(r, v, E2, precision="fp64") as h:
+        h.init_acc_jrk()
+        atime = h.init_dt()
+        info, st, sn = h.run(0.0, atime, 0.25, 1.0, 5)
+        assert info["steps"] == 5 and info["stopped"] == 3
+        assert np.array_equal(st, ra["step_time"][:5]) and np.array_equal(sn, ra["step_nact"][:5])
+        info, st, _ = h.run(info["itime"], info["atime"], 0.25, 0.0625, 1 << 16)
+        assert info["stopped"] == 1 and st[-1] >= 0.0625 and np.all(st[:-1] < 0.0625)
+        assert np.array_equal(st, ra["step_time"][5:5 + len(st)])
+
+
 def test_c1_run_fp64_energy_error_matches(oracle):
     """Fast FP64 kernel over the same run: the cumulative energy error at
     t_end (CumE, Logger.cpp:328) agrees with the oracle's to 2 significant
