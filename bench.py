@@ -58,23 +58,28 @@ def full_step_rate(h4, n, steps, warmup):
     return dt, upd, st
 
 
-def block_schedule_rate(h4, n, steps):
-    """First `steps` block steps of the real schedule from the IC."""
+def block_schedule_rate(h4, n, steps, device_loop):
+    """First `steps` block steps of the real schedule from the IC, either
+    replayed on the device (h4h_run) or one h4h_step call per step."""
     h4.init_acc_jrk()
     T = h4.init_dt()
     h4.synchronize()
     h4.stats(reset=True)
     t0 = time.perf_counter()
-    pairs = 0
-    nacts = []
-    for _ in range(steps):
-        nact, Tn = h4.step(T)
-        pairs += nact * n
-        nacts.append(nact)
-        T = Tn
+    if device_loop:
+        info, _, nacts = h4.run(0.0, T, float("inf"), float("inf"), steps)
+        pairs = info["interactions"]
+        T = info["atime"]
+    else:
+        pairs = 0
+        nacts = []
+        for _ in range(steps):
+            nact, T = h4.step(T)
+            pairs += nact * n
+            nacts.append(nact)
     h4.synchronize()
     el = time.perf_counter() - t0
-    return pairs / el, el, nacts, T
+    return pairs / el, el, np.asarray(nacts), T
 
 
 def nact_sweep(h4, n, nacts, reps=3):
@@ -185,12 +190,15 @@ def main():
 
     sched = None
     if args.block_steps > 0:
-        log(f"[bench] real block-timestep schedule, {args.block_steps} block steps")
-        with Hermite4HIP(r, v, e2, precision=args.precision) as h:
-            rate, el, nacts, T = block_schedule_rate(h, n, args.block_steps)
-        sched = {"pairs_per_s": rate, "block_steps": args.block_steps, "seconds": el,
-                 "t_reached": T, "mean_nact": float(np.mean(nacts)),
-                 "median_nact": float(np.median(nacts))}
+        sched = {}
+        for mode, dev in (("device_loop", True), ("host_loop", False)):
+            log(f"[bench] real block-timestep schedule, {args.block_steps} block steps, {mode}")
+            with Hermite4HIP(r, v, e2, precision=args.precision) as h:
+                rate, el, nacts, T = block_schedule_rate(h, n, args.block_steps, dev)
+            sched[mode] = {"pairs_per_s": rate, "block_steps": args.block_steps, "seconds": el,
+                           "us_per_step": el / args.block_steps * 1e6, "t_reached": T,
+                           "mean_nact": float(np.mean(nacts)),
+                           "median_nact": float(np.median(nacts))}
 
     cpu = None
     if args.cpu_sample > 0:

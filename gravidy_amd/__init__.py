@@ -9,5 +9,6 @@ ABI, used by the tests and bench.py, and the initial-condition tools.
 from . import ic  # noqa: F401
 from ._lib import (H4H_FP64, H4H_FP64_EXACT, H4H_MIXED, H4HError, LIB_PATH,  # noqa: F401
                    load)
+from .ensemble import EnsembleHIP  # noqa: F401
 from .hermite4 import (D_TIME_MAX, D_TIME_MIN, ETA_DEFAULT, Hermite4HIP,  # noqa: F401
                        softening_e2)

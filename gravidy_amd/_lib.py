@@ -31,7 +31,8 @@ EXPORTS = (
     "h4h_get_active", "h4h_save_old_acc_jrk", "h4h_predicted_pos_vel",
     "h4h_update_acc_jrk", "h4h_correction_pos_vel",
     "h4h_next_integration_time", "h4h_step", "h4h_run", "h4h_get_energy",
-    "h4h_set_timing", "h4h_get_stats", "h4h_synchronize",
+    "h4h_set_timing", "h4h_get_stats", "h4h_synchronize", "h4h_create_ensemble",
+    "h4h_ensemble_sizes", "h4h_ensemble_init", "h4h_ensemble_run", "h4h_ensemble_energy",
 )
 
 
@@ -54,6 +55,17 @@ class H4HRunInfo(ctypes.Structure):
         ("interactions", ctypes.c_uint64),
         ("nsteps", ctypes.c_uint64),
         ("last_nact", ctypes.c_uint32),
+        ("stopped", ctypes.c_int),
+    ]
+
+
+class H4HClusterInfo(ctypes.Structure):
+    _fields_ = [
+        ("itime", ctypes.c_double),
+        ("atime", ctypes.c_double),
+        ("steps", ctypes.c_uint64),
+        ("interactions", ctypes.c_uint64),
+        ("nsteps", ctypes.c_uint64),
         ("stopped", ctypes.c_int),
     ]
 
@@ -115,6 +127,11 @@ def load():
         "h4h_set_timing": ([P, i32], i32),
         "h4h_get_stats": ([P, ctypes.POINTER(H4HStats), i32], i32),
         "h4h_synchronize": ([P], i32),
+        "h4h_create_ensemble": ([pp, u32, pu, dbl, flt, dbl, dbl, i32], i32),
+        "h4h_ensemble_sizes": ([P, pu, pu], i32),
+        "h4h_ensemble_init": ([P, flt], i32),
+        "h4h_ensemble_run": ([P, dbl, dbl, ctypes.c_uint64, ctypes.POINTER(H4HClusterInfo)], i32),
+        "h4h_ensemble_energy": ([P, pd, pd], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

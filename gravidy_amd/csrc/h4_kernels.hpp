@@ -38,21 +38,69 @@ struct Soa {
     double *pm;      // predictor mass, float-rounded (Predictor::m)
 };
 
-// Device-side scalars of the block-step loop.  Every step kernel reads the
-// block time and the active count from here, so a step (or a captured
-// chain of steps) needs no host round trip.
-struct DevInfo {
-    uint32_t nact;                    // active count (compaction target)
-    uint32_t done;                    // 1: the step loop has stopped
-    unsigned long long tmin_key;      // order-preserving key of min(t+dt)
-    double itime;                     // ITIME: block time of the current step
+// Device-side scalars of the block-step loop of one cluster (one
+// independent N-body system; a plain context is one cluster).  Every step
+// kernel reads the block time and the active count from here, so a step
+// (or a captured chain of steps) needs no host round trip.
+struct ClState {
+    double itime;                     // ITIME: block time of the last step
     double atime;                     // ATIME: next block time
-    double t_end;                     // steps run while itime < t_end
-    double t_stop;                    // stop after a step with itime >= t_stop
+    uint32_t nact;                    // active count (compaction target)
+    uint32_t done;                    // 1: this cluster's loop has stopped
+    unsigned long long tmin_key;      // order-preserving key of min(t+dt)
     unsigned long long steps;         // block steps since the last setup
-    unsigned long long max_steps;     // stop after this many steps
     unsigned long long interactions;  // sum of nact * n
     unsigned long long nsteps;        // sum of nact
+};
+
+// Work plan of one step for one cluster (computed on the device after the
+// active sets are known).
+struct ClPlan {
+    uint32_t iw;        // i slots per workgroup (lane-group width)
+    uint32_t S;         // j chunks
+    uint32_t tpc;       // JT tiles per chunk
+    uint32_t n_it;      // i tiles
+    uint32_t rows;      // n_it * iw
+    uint32_t tree;      // finish mode: 1 = one workgroup per particle
+    uint32_t grid;      // workgroup share given to decompose()
+    uint32_t pad;
+};
+
+// Loop control shared by all clusters.
+struct RunCtl {
+    double t_end;                  // steps run while itime < t_end
+    double t_stop;                 // stop after a step with itime >= t_stop
+    unsigned long long max_steps;  // stop after this many steps
+    uint32_t ticket;               // last-workgroup detection in the select pass
+    uint32_t ndirty;               // blocks with active particles this step
+    uint32_t ndone;                // clusters that have stopped
+    uint32_t pad;
+    unsigned long long items;      // update work items this step
+    unsigned long long fin_units;  // finish workgroup units this step
+};
+
+// Cluster tables and per-step scratch, by pointer (device memory).
+struct Ens {
+    uint32_t K;              // clusters
+    uint32_t nblk;           // JT blocks of the padded state
+    const uint32_t *off;     // [K] first device index (multiple of JT)
+    const uint32_t *cnt;     // [K] particles
+    const uint32_t *hoff;    // [K] first host index (clusters back to back)
+    const uint32_t *bcl;     // [nblk] cluster of each block
+    ClState *cs;             // [K]
+    ClPlan *plan;            // [K]
+    unsigned long long *items_pre;  // [K+1] exclusive prefix of work items
+    unsigned long long *fin_pre;    // [K+1] exclusive prefix of finish units
+    unsigned long long *part_pre;   // [K+1] exclusive prefix of partial doubles
+    RunCtl *ctl;
+    uint32_t *dirty;         // [nblk] dirty block list
+    double *bmin;            // [nblk] min(t + dt) per block
+    double *hist_t;          // per-step history of cluster 0 (ring)
+    uint32_t *hist_n;
+    uint32_t hist_cap;
+    uint32_t grid;           // persistent workgroups of the update kernel
+    uint32_t max_split;
+    uint32_t pad;
 };
 
 // Order-preserving map double -> uint64 for atomicMin.

@@ -52,165 +52,235 @@ constexpr uint32_t DEFAULT_MAX_SPLIT = 1024;  // largest j split
 constexpr uint32_t ENERGY_GRID = 1024;
 
 // ---------------------------------------------------------------------------
-// Layout kernels
+// Cluster geometry helpers
+// ---------------------------------------------------------------------------
+//
+// A context holds K independent clusters (K = 1 for a single system).
+// Cluster c occupies device indices off[c] .. off[c] + cnt[c] - 1, padded
+// to a multiple of JT with zero-mass particles, so every JT block belongs
+// to exactly one cluster (bcl[block]) and j tiles never straddle clusters.
+// Host arrays hold the clusters back to back (hoff[c]).
+
+__device__ __forceinline__ uint32_t cluster_of_block(const Ens &e, uint32_t b) { return e.bcl[b]; }
+
+__device__ __forceinline__ uint32_t ntiles(const Ens &e, uint32_t c) {
+    return (e.cnt[c] + JT - 1) / JT;
+}
+
+// Index of the cluster whose [pre[c], pre[c+1]) range holds x (pre has
+// K+1 entries, non-decreasing, pre[0] = 0, x < pre[K]).
+__device__ __forceinline__ uint32_t find_cluster(const unsigned long long *pre, uint32_t K,
+                                                 unsigned long long x) {
+    uint32_t lo = 0, hi = K;  // answer in [lo, hi)
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Layout kernels (one thread per padded device index)
 // ---------------------------------------------------------------------------
 
 struct Cols {
     double *c[8];
 };
 
-// dst.c[k][i] = src[i*ncols + k] for k < ncols (NULL columns skipped).
-__global__ __launch_bounds__(BS) void k_aos_to_soa(const double *src,
-                                                  uint32_t ncols, Cols dst,
-                                                  uint32_t n) {
+// dst.c[k][i] = src[h*ncols + k] for the host index h of device index i.
+__global__ __launch_bounds__(BS) void k_aos_to_soa(const double *src, uint32_t ncols, Cols dst,
+                                                  Ens e) {
     uint32_t i = blockIdx.x * BS + threadIdx.x;
-    if (i >= n) return;
+    if (i >= e.nblk * JT) return;
+    uint32_t c = cluster_of_block(e, i / JT);
+    uint32_t local = i - e.off[c];
+    if (local >= e.cnt[c]) return;
+    size_t h = (size_t)e.hoff[c] + local;
     for (uint32_t k = 0; k < ncols; ++k)
-        if (dst.c[k]) dst.c[k][i] = src[(size_t)i * ncols + k];
+        if (dst.c[k]) dst.c[k][i] = src[h * ncols + k];
 }
 
-// dst[i*ncols + k] = src.c[k][i] (NULL columns write 0).
-__global__ __launch_bounds__(BS) void k_soa_to_aos(Cols src, uint32_t ncols,
-                                                  double *dst, uint32_t n) {
+// dst[h*ncols + k] = src.c[k][i] (NULL columns write 0).
+__global__ __launch_bounds__(BS) void k_soa_to_aos(Cols src, uint32_t ncols, double *dst, Ens e) {
     uint32_t i = blockIdx.x * BS + threadIdx.x;
-    if (i >= n) return;
-    for (uint32_t k = 0; k < ncols; ++k)
-        dst[(size_t)i * ncols + k] = src.c[k] ? src.c[k][i] : 0.0;
+    if (i >= e.nblk * JT) return;
+    uint32_t c = cluster_of_block(e, i / JT);
+    uint32_t local = i - e.off[c];
+    if (local >= e.cnt[c]) return;
+    size_t h = (size_t)e.hoff[c] + local;
+    for (uint32_t k = 0; k < ncols; ++k) dst[h * ncols + k] = src.c[k] ? src.c[k][i] : 0.0;
+}
+
+// Same for one array of scalars (t, dt) and for the u32 active list of a
+// single cluster context.
+__global__ __launch_bounds__(BS) void k_scalar_to_soa(const double *src, double *dst, Ens e) {
+    uint32_t i = blockIdx.x * BS + threadIdx.x;
+    if (i >= e.nblk * JT) return;
+    uint32_t c = cluster_of_block(e, i / JT);
+    uint32_t local = i - e.off[c];
+    if (local < e.cnt[c]) dst[i] = src[e.hoff[c] + local];
+}
+
+__global__ __launch_bounds__(BS) void k_soa_to_scalar(const double *src, double *dst, Ens e) {
+    uint32_t i = blockIdx.x * BS + threadIdx.x;
+    if (i >= e.nblk * JT) return;
+    uint32_t c = cluster_of_block(e, i / JT);
+    uint32_t local = i - e.off[c];
+    if (local < e.cnt[c]) dst[e.hoff[c] + local] = src[i];
 }
 
 // Predictor at t_i (Hermite4::init_data, Hermite4.cpp:194-240): p = r, v;
-// pm = (float)m.  Padding slots n..n_pad get zero mass far away.
-__global__ __launch_bounds__(BS) void k_init_predictor(Soa s, uint32_t n,
-                                                      uint32_t n_pad) {
+// pm = (float)m.  Padding slots get zero mass far away and never become
+// active (t = 0, dt = +inf).
+__global__ __launch_bounds__(BS) void k_init_predictor(Soa s, Ens e) {
     uint32_t i = blockIdx.x * BS + threadIdx.x;
-    if (i >= n_pad) return;
-    if (i < n) {
-        for (int c = 0; c < 3; ++c) {
-            s.p[c][i] = s.r[c][i];
-            s.p[3 + c][i] = s.v[c][i];
+    if (i >= e.nblk * JT) return;
+    uint32_t c = cluster_of_block(e, i / JT);
+    if (i - e.off[c] < e.cnt[c]) {
+        for (int k = 0; k < 3; ++k) {
+            s.p[k][i] = s.r[k][i];
+            s.p[3 + k][i] = s.v[k][i];
         }
         s.pm[i] = (double)(float)s.m[i];
     } else {
-        for (int c = 0; c < 3; ++c) {
-            s.p[c][i] = PAD_POS;
-            s.p[3 + c][i] = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            s.p[k][i] = PAD_POS;
+            s.p[3 + k][i] = 0.0;
         }
         s.pm[i] = 0.0;
+        s.dt[i] = INFINITY;
     }
 }
 
-// ---------------------------------------------------------------------------
-// Step-loop control (one thread)
-// ---------------------------------------------------------------------------
-
-// Arms the device step loop (h4h_run / h4h_step).
-__global__ void k_run_setup(DevInfo *info, double itime, double atime, double t_end,
-                            double t_stop, unsigned long long max_steps) {
-    info->done = 0;
-    info->itime = itime;
-    info->atime = atime;
-    info->t_end = t_end;
-    info->t_stop = t_stop;
-    info->steps = 0;
-    info->max_steps = max_steps;
-    info->interactions = 0;
-    info->nsteps = 0;
-}
-
-// Loop head of Hermite4CPU::integration (Hermite4CPU_integration.cpp:81-83):
-// stop unless ITIME < t_end, then ITIME = ATIME; reset the active count
-// and the next-time minimum.
-__global__ void k_step_begin(DevInfo *info) {
-    if (info->done) return;
-    if (!(info->itime < info->t_end)) {
-        info->done = 1;
-        return;
-    }
-    info->itime = info->atime;
-    info->nact = 0;
-    info->tmin_key = dkey(INFINITY);
-}
-
-// Loop tail: ATIME = min(t+dt), counters, per-step history, and the stop
-// test of the output interval (Hermite4CPU_integration.cpp:96-102).
-__global__ void k_step_end(DevInfo *info, uint32_t n, double *hist_t, uint32_t *hist_n,
-                           uint32_t hist_cap) {
-    if (info->done) return;
-    info->atime = dkey_inv(info->tmin_key);
-    unsigned long long k = info->steps;
-    hist_t[k % hist_cap] = info->itime;
-    hist_n[k % hist_cap] = info->nact;
-    info->steps = k + 1;
-    info->interactions += (unsigned long long)info->nact * n;
-    info->nsteps += info->nact;
-    if (info->itime >= info->t_stop || info->steps >= info->max_steps) info->done = 1;
-}
-
-// Single-method ABI calls: set the block time / reset counters.
-__global__ void k_set_time(DevInfo *info, double itime) {
-    info->done = 0;
-    info->itime = itime;
-}
-
-__global__ void k_begin_select(DevInfo *info) {
-    info->done = 0;
-    info->nact = 0;
-}
-
-__global__ void k_reset_tmin(DevInfo *info) {
-    info->done = 0;
-    info->tmin_key = dkey(INFINITY);
-}
-
-__global__ void k_set_nact(DevInfo *info, uint32_t nact) {
-    info->done = 0;
-    info->nact = nact;
-}
-
-__global__ __launch_bounds__(BS) void k_identity_active(uint32_t *move,
-                                                       uint32_t n) {
+// Active list of every cluster = all its particles (init_acc_jrk).
+__global__ __launch_bounds__(BS) void k_identity_active(uint32_t *move, Ens e) {
     uint32_t i = blockIdx.x * BS + threadIdx.x;
-    if (i < n) move[i] = i;
+    if (i >= e.nblk * JT) return;
+    uint32_t c = cluster_of_block(e, i / JT);
+    uint32_t local = i - e.off[c];
+    if (local < e.cnt[c]) move[i] = i;
+    if (local == 0) e.cs[c].nact = e.cnt[c];
 }
 
 // ---------------------------------------------------------------------------
-// Predictor / active selection / save old (all n, O(n))
+// Work plan of a step (one workgroup)
 // ---------------------------------------------------------------------------
 
-enum : uint32_t { PS_PREDICT = 1, PS_SELECT = 2, PS_SAVE_OLD = 4 };
+// Per cluster: lane-group width, j split and work items from its active
+// count; the update workgroups are shared in proportion to nact * n.
+// Exclusive prefix sums over clusters go to items_pre / fin_pre /
+// part_pre, totals to ctl.  `exact`: one lane per i, no j split.
+__device__ void plan_all(const Ens &e, uint32_t exact) {
+    __shared__ double wsum[BS / 64];
+    double w = 0.0;
+    for (uint32_t c = threadIdx.x; c < e.K; c += BS)
+        w += (double)e.cs[c].nact * (double)e.cnt[c];
+    for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+    if (__lane_id() == 0) wsum[threadIdx.x >> 6] = w;
+    __syncthreads();
+    double W = 0.0;
+    for (uint32_t k = 0; k < BS / 64; ++k) W += wsum[k];
+    for (uint32_t c = threadIdx.x; c < e.K; c += BS) {
+        uint32_t nact = e.cs[c].nact;
+        ClPlan pl;
+        pl.pad = 0;
+        if (exact) {
+            pl.iw = BS;
+            pl.S = 1;
+            pl.tpc = ntiles(e, c);
+            pl.n_it = (nact + BS - 1) / BS;
+            pl.rows = pl.n_it * BS;
+            pl.tree = 0;
+            pl.grid = e.grid;
+        } else {
+            double share = W > 0.0 ? (double)nact * (double)e.cnt[c] / W : 0.0;
+            uint32_t g = (uint32_t)(share * e.grid);
+            pl.grid = g < 1 ? 1 : g;
+            pl.iw = pick_iw(nact ? nact : 1);
+            Decomp d = decompose(nact, pl.iw, ntiles(e, c), pl.grid, e.max_split);
+            pl.S = d.S;
+            pl.tpc = d.tiles_per_chunk;
+            pl.n_it = d.n_it;
+            pl.rows = d.rows;
+            pl.tree = d.S > 32 ? 1 : 0;
+        }
+        e.plan[c] = pl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long it = 0, fu = 0, pt = 0;
+        for (uint32_t c = 0; c < e.K; ++c) {
+            e.items_pre[c] = it;
+            e.fin_pre[c] = fu;
+            e.part_pre[c] = pt;
+            uint32_t nact = e.cs[c].nact;
+            if (nact) {
+                ClPlan pl = e.plan[c];
+                it += (unsigned long long)pl.n_it * pl.S;
+                fu += pl.tree ? nact : (nact + BS - 1) / BS;
+                if (pl.S > 1) pt += 6ull * pl.S * pl.rows;
+            }
+        }
+        e.items_pre[e.K] = it;
+        e.fin_pre[e.K] = fu;
+        e.part_pre[e.K] = pt;
+        e.ctl->items = it;
+        e.ctl->fin_units = fu;
+    }
+}
 
-// PS_PREDICT: Hermite4CPU::predicted_pos_vel (Hermite4CPU.cpp:340-365).
-// PS_SELECT:  Hermite4::find_particles_to_move (Hermite4.cpp:63-84): the
-//             active set is appended to move[] with one atomic per
-//             workgroup (ballot per wave, prefix over the waves in LDS).
-// PS_SAVE_OLD: Hermite4::save_old_acc_jrk (Hermite4.cpp:145-153).
-// The block time is info->itime.
-__global__ __launch_bounds__(BS) void k_predict_select(Soa s, uint32_t n, uint32_t flags,
-                                                      uint32_t *move, DevInfo *info) {
+__global__ __launch_bounds__(BS) void k_plan(Ens e, uint32_t exact) { plan_all(e, exact); }
+
+// ---------------------------------------------------------------------------
+// Predictor / active selection / save old (one workgroup per JT block)
+// ---------------------------------------------------------------------------
+
+enum : uint32_t {
+    PS_PREDICT = 1,   // Hermite4CPU::predicted_pos_vel (Hermite4CPU.cpp:340-365)
+    PS_SELECT = 2,    // Hermite4::find_particles_to_move (Hermite4.cpp:63-84)
+    PS_SAVE_OLD = 4,  // Hermite4::save_old_acc_jrk (Hermite4.cpp:145-153)
+    PS_DEVTIME = 8,   // block time = the cluster's ATIME (device step loop)
+    PS_DIRTY = 16,    // record blocks holding active particles
+    PS_PLAN = 32,     // the last workgroup computes the step's work plan
+    PS_EXACT = 64,    // plan for the FP64_EXACT kernel
+};
+
+// The active set of a cluster is appended to its segment of move[] with one
+// atomic per workgroup (ballot per wave, prefix over the waves in LDS).
+// Clusters whose loop has stopped are skipped.
+__global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, double itime_arg,
+                                              uint32_t *move) {
 #pragma clang fp contract(off)
     __shared__ uint32_t wave_off[BS / 64 + 1];
-    if (info->done) return;
-    const double itime = info->itime;
-    uint32_t i = blockIdx.x * BS + threadIdx.x;
-    bool in = i < n;
+    __shared__ uint32_t is_last;
+    const uint32_t b = blockIdx.x;
+    const uint32_t c = cluster_of_block(e, b);
+    ClState *cs = &e.cs[c];
+    const bool live = (flags & PS_DEVTIME) ? cs->done == 0 : true;
+    const double itime = (flags & PS_DEVTIME) ? cs->atime : itime_arg;
+    const uint32_t i = b * JT + threadIdx.x;
+    const bool in = live && (i - e.off[c]) < e.cnt[c];
     bool act = false;
     if (in) {
         double ti = s.t[i];
         if (flags & PS_PREDICT) {
             double f[6], r[3], v[3];
-            for (int c = 0; c < 6; ++c) f[c] = s.f[c][i];
-            for (int c = 0; c < 3; ++c) {
-                r[c] = s.r[c][i];
-                v[c] = s.v[c][i];
+            for (int k = 0; k < 6; ++k) f[k] = s.f[k][i];
+            for (int k = 0; k < 3; ++k) {
+                r[k] = s.r[k][i];
+                v[k] = s.v[k][i];
             }
             double dt = itime - ti;
             double dt2 = (dt * dt);
             double dt3 = (dt2 * dt);
             double dt2c = dt2 / 2.0;
             double dt3c = dt3 / 6.0;
-            for (int c = 0; c < 3; ++c) {
-                s.p[c][i] = (dt3c * f[3 + c]) + (dt2c * f[c]) + (dt * v[c]) + r[c];
-                s.p[3 + c][i] = (dt2c * f[3 + c]) + (dt * f[c]) + v[c];
+            for (int k = 0; k < 3; ++k) {
+                s.p[k][i] = (dt3c * f[3 + k]) + (dt2c * f[k]) + (dt * v[k]) + r[k];
+                s.p[3 + k][i] = (dt2c * f[3 + k]) + (dt * f[k]) + v[k];
             }
             s.pm[i] = (double)(float)s.m[i];
         }
@@ -228,29 +298,43 @@ __global__ __launch_bounds__(BS) void k_predict_select(Soa s, uint32_t n, uint32
         if (threadIdx.x == 0) {
             uint32_t total = 0;
             for (uint32_t k = 0; k < BS / 64; ++k) {
-                uint32_t c = wave_off[k];
+                uint32_t n = wave_off[k];
                 wave_off[k] = total;
-                total += c;
+                total += n;
             }
-            wave_off[BS / 64] = total ? atomicAdd(&info->nact, total) : 0;
+            wave_off[BS / 64] = total ? atomicAdd(&cs->nact, total) : 0;
+            if (total && (flags & PS_DIRTY)) e.dirty[atomicAdd(&e.ctl->ndirty, 1u)] = b;
         }
         __syncthreads();
         if (act) {
             unsigned long long below = mask & ((1ull << lane) - 1ull);
-            move[wave_off[BS / 64] + wave_off[w] + (uint32_t)__popcll(below)] = i;
+            move[e.off[c] + wave_off[BS / 64] + wave_off[w] + (uint32_t)__popcll(below)] = i;
             if (flags & PS_SAVE_OLD)
-                for (int c = 0; c < 6; ++c) s.old[c][i] = s.f[c][i];
+                for (int k = 0; k < 6; ++k) s.old[k][i] = s.f[k][i];
+        }
+    }
+    if (flags & PS_PLAN) {
+        // Last workgroup to finish computes the plan from the final counts.
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0)
+            is_last = atomicAdd(&e.ctl->ticket, 1u) == gridDim.x - 1 ? 1u : 0u;
+        __syncthreads();
+        if (is_last) {
+            __threadfence();
+            plan_all(e, (flags & PS_EXACT) ? 1u : 0u);
+            if (threadIdx.x == 0) e.ctl->ticket = 0;
         }
     }
 }
 
-__global__ __launch_bounds__(BS) void k_save_old(Soa s, const uint32_t *move,
-                                                const DevInfo *info) {
-    uint32_t nact = info->nact;
-    for (uint32_t k = blockIdx.x * BS + threadIdx.x; k < nact;
-         k += gridDim.x * BS) {
-        uint32_t i = move[k];
-        for (int c = 0; c < 6; ++c) s.old[c][i] = s.f[c][i];
+__global__ __launch_bounds__(BS) void k_save_old(Soa s, Ens e, const uint32_t *move) {
+    for (uint32_t c = 0; c < e.K; ++c) {
+        uint32_t nact = e.cs[c].nact;
+        for (uint32_t k = blockIdx.x * BS + threadIdx.x; k < nact; k += gridDim.x * BS) {
+            uint32_t i = move[e.off[c] + k];
+            for (int q = 0; q < 6; ++q) s.old[q][i] = s.f[q][i];
+        }
     }
 }
 
@@ -274,42 +358,6 @@ __device__ __forceinline__ void load_jtile(const Soa &s, uint32_t j0,
 
 __device__ __forceinline__ float mass_f32(const double4 &b) {
     return __builtin_bit_cast(float, (uint32_t)__builtin_bit_cast(unsigned long long, b.w));
-}
-
-// FP64_EXACT: one lane per active i, j = 0..n-1 ascending, j == i skipped,
-// exactly Hermite4CPU::update_acc_jrk (Hermite4CPU.cpp:309-336).  Writes
-// f[i] directly (no j split).  Workgroups stride over i tiles.
-__global__ __launch_bounds__(BS) void k_update_exact(Soa s,
-                                                    const uint32_t *move,
-                                                    const DevInfo *info,
-                                                    uint32_t n, double e2) {
-    __shared__ double4 sp[JT];
-    __shared__ double4 sv[JT];
-    if (info->done) return;
-    const uint32_t nact = info->nact;
-    const uint32_t n_it = (nact + BS - 1) / BS;
-    for (uint32_t it = blockIdx.x; it < n_it; it += gridDim.x) {
-        uint32_t k = it * BS + threadIdx.x;
-        bool act = k < nact;
-        uint32_t i = act ? move[k] : 0;
-        double pi[6];
-        for (int c = 0; c < 6; ++c) pi[c] = s.p[c][i];
-        double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (uint32_t j0 = 0; j0 < n; j0 += JT) {
-            __syncthreads();
-            load_jtile(s, j0, sp, sv);
-            __syncthreads();
-            uint32_t jn = min((uint32_t)JT, n - j0);
-            for (uint32_t jj = 0; jj < jn; ++jj) {
-                double4 a = sp[jj];
-                double4 b = sv[jj];
-                if (j0 + jj != i)
-                    pair_exact(pi, a.x, a.y, a.z, b.x, b.y, b.z, a.w, e2, fo);
-            }
-        }
-        if (act)
-            for (int c = 0; c < 6; ++c) s.f[c][i] = fo[c];
-    }
 }
 
 // Inner loop of one lane over one LDS tile: the lane's j sub-group js
@@ -399,133 +447,124 @@ __device__ __forceinline__ void reduce_subgroups(double fo[6], double *red) {
     }
 }
 
-// FP64 and mixed precision: persistent workgroups over (i tile, j chunk)
-// items.  With one chunk the result goes to f[i]; with several, chunk
-// partial sums go to partial[c][chunk][k] and k_reduce_partials adds them
-// in chunk order.
-template <int IW, bool MIXED>
-__device__ __forceinline__ void update_body(const Soa &s, const uint32_t *move, uint32_t nact,
-                                            double *partial, uint32_t n_jt,
-                                            uint32_t max_split, double e2, double4 *sp,
-                                            double4 *sv, double *red) {
-    const Decomp d = decompose(nact, IW, n_jt, gridDim.x, max_split);
-    const uint32_t il = threadIdx.x % IW;
-    const uint32_t js = threadIdx.x / IW;
-    for (uint32_t item = blockIdx.x; item < d.n_items; item += gridDim.x) {
-        uint32_t it = item / d.S;
-        uint32_t sc = item - it * d.S;
-        uint32_t k = it * IW + il;
+// FP64_EXACT: one lane per active i, j ascending over the cluster, j == i
+// skipped, exactly Hermite4CPU::update_acc_jrk (Hermite4CPU.cpp:309-336).
+// Writes f[i] directly.  Workgroups stride over the plan's i tiles.
+__global__ __launch_bounds__(BS) void k_update_exact(Soa s, Ens e, const uint32_t *move,
+                                                    double e2) {
+    __shared__ double4 sp[JT];
+    __shared__ double4 sv[JT];
+    const unsigned long long total = e.ctl->items;
+    for (unsigned long long item = blockIdx.x; item < total; item += gridDim.x) {
+        const uint32_t c = find_cluster(e.items_pre, e.K, item);
+        const uint32_t it = (uint32_t)(item - e.items_pre[c]);
+        const uint32_t nact = e.cs[c].nact;
+        const uint32_t *mv = move + e.off[c];
+        const uint32_t j_begin = e.off[c], j_end = e.off[c] + e.cnt[c];
+        uint32_t k = it * BS + threadIdx.x;
         bool act = k < nact;
-        uint32_t i = act ? move[k] : move[0];
+        uint32_t i = act ? mv[k] : mv[0];
         double pi[6];
-        for (int c = 0; c < 6; ++c) pi[c] = s.p[c][i];
+        for (int q = 0; q < 6; ++q) pi[q] = s.p[q][i];
         double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        uint32_t t0 = sc * d.tiles_per_chunk;
-        uint32_t t1 = min(n_jt, t0 + d.tiles_per_chunk);
-        for (uint32_t tt = t0; tt < t1; ++tt) {
-            uint32_t j0 = tt * JT;
+        for (uint32_t j0 = j_begin; j0 < j_end; j0 += JT) {
             __syncthreads();
             load_jtile(s, j0, sp, sv);
             __syncthreads();
-            bool self = (i - j0) < (uint32_t)JT;
-            if (__ballot(self))
-                tile_pairs<IW, true, MIXED>(pi, i, j0, js, sp, sv, e2, fo);
-            else
-                tile_pairs<IW, false, MIXED>(pi, i, j0, js, sp, sv, e2, fo);
-        }
-        reduce_subgroups<IW>(fo, red);
-        if (threadIdx.x < IW && act) {
-            if (d.S == 1) {
-                for (int c = 0; c < 6; ++c) s.f[c][i] = fo[c];
-            } else {
-                size_t plane = (size_t)d.S * d.rows;
-                size_t off = (size_t)sc * d.rows + k;
-                for (int c = 0; c < 6; ++c) partial[c * plane + off] = fo[c];
+            uint32_t jn = min((uint32_t)JT, j_end - j0);
+            for (uint32_t jj = 0; jj < jn; ++jj) {
+                double4 a = sp[jj];
+                double4 b = sv[jj];
+                if (j0 + jj != i)
+                    pair_exact(pi, a.x, a.y, a.z, b.x, b.y, b.z, a.w, e2, fo);
             }
+        }
+        if (act)
+            for (int q = 0; q < 6; ++q) s.f[q][i] = fo[q];
+    }
+}
+
+// One work item of the FP64 / mixed update: i tile `it` of a cluster
+// against j chunk `sc`.  With one chunk the result goes to f[i]; with
+// several, the chunk sums go to partial[(comp*S + sc)*rows + k] and the
+// finish pass adds them in chunk order.
+template <int IW, bool MIXED>
+__device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint32_t jt_begin,
+                                            uint32_t jt_end, uint32_t it, uint32_t sc,
+                                            const uint32_t *mv, uint32_t nact, double *partial,
+                                            double e2, double4 *sp, double4 *sv, double *red) {
+    const uint32_t il = threadIdx.x % IW;
+    const uint32_t js = threadIdx.x / IW;
+    uint32_t k = it * IW + il;
+    bool act = k < nact;
+    uint32_t i = act ? mv[k] : mv[0];
+    double pi[6];
+    for (int q = 0; q < 6; ++q) pi[q] = s.p[q][i];
+    double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    uint32_t t0 = jt_begin + sc * pl.tpc;
+    uint32_t t1 = min(jt_end, t0 + pl.tpc);
+    for (uint32_t tt = t0; tt < t1; ++tt) {
+        uint32_t j0 = tt * JT;
+        __syncthreads();
+        load_jtile(s, j0, sp, sv);
+        __syncthreads();
+        bool self = (i - j0) < (uint32_t)JT;
+        if (__ballot(self))
+            tile_pairs<IW, true, MIXED>(pi, i, j0, js, sp, sv, e2, fo);
+        else
+            tile_pairs<IW, false, MIXED>(pi, i, j0, js, sp, sv, e2, fo);
+    }
+    reduce_subgroups<IW>(fo, red);
+    if (threadIdx.x < IW && act) {
+        if (pl.S == 1) {
+            for (int q = 0; q < 6; ++q) s.f[q][i] = fo[q];
+        } else {
+            for (int q = 0; q < 6; ++q)
+                partial[((size_t)q * pl.S + sc) * pl.rows + k] = fo[q];
         }
     }
 }
 
-// One kernel for every active-set size: the lane-group width IW is picked
-// on the device from nact (pick_iw), so the launch needs no host knowledge
-// of nact.
+// FP64 and mixed precision: persistent workgroups over the plan's work
+// items of all clusters; each item's lane-group width comes from the plan,
+// so one launch serves every active-set size.
 template <bool MIXED>
-__global__ __launch_bounds__(BS) void k_update_fp64(Soa s, const uint32_t *move,
-                                                   const DevInfo *info, double *partial,
-                                                   uint32_t n_jt, uint32_t max_split,
-                                                   double e2) {
+__global__ __launch_bounds__(BS) void k_update_fp64(Soa s, Ens e, const uint32_t *move,
+                                                   double *partial, double e2) {
     __shared__ double4 sp[JT];
     __shared__ double4 sv[JT];
     __shared__ double red[BS * 6];
-    if (info->done) return;
-    const uint32_t nact = info->nact;
-    if (nact == 0) return;
-    switch (pick_iw(nact)) {
-        case 4:
-            update_body<4, MIXED>(s, move, nact, partial, n_jt, max_split, e2, sp, sv, red);
-            break;
-        case 16:
-            update_body<16, MIXED>(s, move, nact, partial, n_jt, max_split, e2, sp, sv, red);
-            break;
-        case 64:
-            update_body<64, MIXED>(s, move, nact, partial, n_jt, max_split, e2, sp, sv, red);
-            break;
-        default:
-            update_body<256, MIXED>(s, move, nact, partial, n_jt, max_split, e2, sp, sv, red);
-            break;
-    }
-}
-
-// Adds the S chunk partials of every active particle in chunk order.
-// Few chunks: one lane per particle.  Many chunks: one workgroup per
-// particle, lanes striding over chunks, then a fixed-order tree.
-__global__ __launch_bounds__(BS) void k_reduce_partials(Soa s,
-                                                       const uint32_t *move,
-                                                       const DevInfo *info,
-                                                       const double *partial,
-                                                       uint32_t n_jt,
-                                                       uint32_t grid,
-                                                       uint32_t max_split) {
-    __shared__ double red[BS / 64];
-    if (info->done) return;
-    const uint32_t nact = info->nact;
-    if (nact == 0) return;
-    const Decomp d = decompose(nact, pick_iw(nact), n_jt, grid, max_split);
-    if (d.S == 1) return;
-    size_t plane = (size_t)d.S * d.rows;
-    if (d.S <= 32) {
-        for (uint32_t k = blockIdx.x * BS + threadIdx.x; k < nact; k += gridDim.x * BS) {
-            uint32_t i = move[k];
-            for (int c = 0; c < 6; ++c) {
-                double acc = 0.0;
-                for (uint32_t sc = 0; sc < d.S; ++sc)
-                    acc += partial[c * plane + (size_t)sc * d.rows + k];
-                s.f[c][i] = acc;
-            }
-        }
-        return;
-    }
-    for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
-        uint32_t i = move[k];
-        for (int c = 0; c < 6; ++c) {
-            double acc = 0.0;
-            for (uint32_t sc = threadIdx.x; sc < d.S; sc += BS)
-                acc += partial[c * plane + (size_t)sc * d.rows + k];
-            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-            __syncthreads();
-            if (__lane_id() == 0) red[threadIdx.x >> 6] = acc;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double t = red[0];
-                for (uint32_t w = 1; w < BS / 64; ++w) t += red[w];
-                s.f[c][i] = t;
-            }
+    const unsigned long long total = e.ctl->items;
+    for (unsigned long long item = blockIdx.x; item < total; item += gridDim.x) {
+        const uint32_t c = find_cluster(e.items_pre, e.K, item);
+        const ClPlan pl = e.plan[c];
+        const uint32_t li = (uint32_t)(item - e.items_pre[c]);
+        const uint32_t it = li / pl.S;
+        const uint32_t sc = li - it * pl.S;
+        const uint32_t nact = e.cs[c].nact;
+        const uint32_t jt_begin = e.off[c] / JT;
+        const uint32_t jt_end = jt_begin + ntiles(e, c);
+        const uint32_t *mv = move + e.off[c];
+        double *part = partial + e.part_pre[c];
+        switch (pl.iw) {
+            case 4:
+                update_item<4, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, sp, sv, red);
+                break;
+            case 16:
+                update_item<16, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, sp, sv, red);
+                break;
+            case 64:
+                update_item<64, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, sp, sv, red);
+                break;
+            default:
+                update_item<256, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, sp, sv, red);
+                break;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Corrector + Aarseth timestep + block quantisation (active only)
+// Corrector + Aarseth timestep + block quantisation
 // ---------------------------------------------------------------------------
 
 // NbodyUtils::get_magnitude (NbodyUtils.cpp:331-334).
@@ -563,56 +602,125 @@ __device__ __forceinline__ double normalize_dt(double new_dt, double old_dt,
 }
 
 // Hermite4CPU::correction_pos_vel (Hermite4CPU.cpp:370-411) with
-// NbodyUtils::get_timestep_normal (NbodyUtils.cpp:337-365).
-__global__ __launch_bounds__(BS) void k_correct(Soa s, const uint32_t *move,
-                                               const DevInfo *info, float eta,
-                                               double dt_min, double dt_max) {
+// NbodyUtils::get_timestep_normal (NbodyUtils.cpp:337-365) for particle i
+// at block time itime, given its new forces f[6] (old forces in s.old,
+// predictor in s.p).
+__device__ __forceinline__ void correct_one(const Soa &s, uint32_t i, const double f[6],
+                                            double itime, float eta, double dt_min,
+                                            double dt_max) {
 #pragma clang fp contract(off)
-    if (info->done) return;
-    const double itime = info->itime;
-    const uint32_t nact = info->nact;
-    for (uint32_t k = blockIdx.x * BS + threadIdx.x; k < nact;
-         k += gridDim.x * BS) {
-        uint32_t i = move[k];
-        double f[6], o[6], a2[3], a3[3];
-        for (int c = 0; c < 6; ++c) {
-            f[c] = s.f[c][i];
-            o[c] = s.old[c][i];
-        }
-        double dt1 = s.dt[i];
-        double dt2 = dt1 * dt1;
-        double dt3 = dt2 * dt1;
-        double dt4 = dt2 * dt2;
-        double dt5 = dt4 * dt1;
-        for (int c = 0; c < 3; ++c)
-            a2[c] = (-6 * (o[c] - f[c]) - dt1 * (4 * o[3 + c] + 2 * f[3 + c])) / dt2;
-        for (int c = 0; c < 3; ++c)
-            a3[c] = (12 * (o[c] - f[c]) + 6 * dt1 * (o[3 + c] + f[3 + c])) / dt3;
-        for (int c = 0; c < 3; ++c) {
-            s.r[c][i] = s.p[c][i] + (dt4 / 24) * a2[c] + (dt5 / 120) * a3[c];
-            s.v[c][i] = s.p[3 + c][i] + (dt3 / 6) * a2[c] + (dt4 / 24) * a3[c];
-            s.a2[c][i] = a2[c];
-            s.a3[c][i] = a3[c];
-        }
-        s.t[i] = itime;
+    double o[6], a2[3], a3[3];
+    for (int c = 0; c < 6; ++c) o[c] = s.old[c][i];
+    double dt1 = s.dt[i];
+    double dt2 = dt1 * dt1;
+    double dt3 = dt2 * dt1;
+    double dt4 = dt2 * dt2;
+    double dt5 = dt4 * dt1;
+    for (int c = 0; c < 3; ++c)
+        a2[c] = (-6 * (o[c] - f[c]) - dt1 * (4 * o[3 + c] + 2 * f[3 + c])) / dt2;
+    for (int c = 0; c < 3; ++c)
+        a3[c] = (12 * (o[c] - f[c]) + 6 * dt1 * (o[3 + c] + f[3 + c])) / dt3;
+    for (int c = 0; c < 3; ++c) {
+        s.r[c][i] = s.p[c][i] + (dt4 / 24) * a2[c] + (dt5 / 120) * a3[c];
+        s.v[c][i] = s.p[3 + c][i] + (dt3 / 6) * a2[c] + (dt4 / 24) * a3[c];
+        s.a2[c][i] = a2[c];
+        s.a3[c][i] = a3[c];
+    }
+    s.t[i] = itime;
 
-        double ax1_2 = a2[0] + dt1 * a3[0];
-        double ay1_2 = a2[1] + dt1 * a3[1];
-        double az1_2 = a2[2] + dt1 * a3[2];
-        double abs_a1 = magnitude(f[0], f[1], f[2]);
-        double abs_j1 = magnitude(f[3], f[4], f[5]);
-        double abs_j12 = abs_j1 * abs_j1;
-        double abs_a1_3 = magnitude(a3[0], a3[1], a3[2]);
-        double abs_a1_2 = magnitude(ax1_2, ay1_2, az1_2);
-        double abs_a1_22 = abs_a1_2 * abs_a1_2;
-        double normal_dt = sqrt(eta * ((abs_a1 * abs_a1_2 + abs_j12) /
-                                       (abs_j1 * abs_a1_3 + abs_a1_22)));
-        s.dt[i] = normalize_dt(normal_dt, dt1, itime, dt_min, dt_max);
+    double ax1_2 = a2[0] + dt1 * a3[0];
+    double ay1_2 = a2[1] + dt1 * a3[1];
+    double az1_2 = a2[2] + dt1 * a3[2];
+    double abs_a1 = magnitude(f[0], f[1], f[2]);
+    double abs_j1 = magnitude(f[3], f[4], f[5]);
+    double abs_j12 = abs_j1 * abs_j1;
+    double abs_a1_3 = magnitude(a3[0], a3[1], a3[2]);
+    double abs_a1_2 = magnitude(ax1_2, ay1_2, az1_2);
+    double abs_a1_22 = abs_a1_2 * abs_a1_2;
+    double normal_dt = sqrt(eta * ((abs_a1 * abs_a1_2 + abs_j12) /
+                                   (abs_j1 * abs_a1_3 + abs_a1_22)));
+    s.dt[i] = normalize_dt(normal_dt, dt1, itime, dt_min, dt_max);
+}
+
+// Standalone corrector of cluster 0's active list at itime (the ABI call
+// h4h_correction_pos_vel): forces from s.f.
+__global__ __launch_bounds__(BS) void k_correct(Soa s, Ens e, const uint32_t *move, double itime,
+                                               float eta, double dt_min, double dt_max) {
+    const uint32_t nact = e.cs[0].nact;
+    for (uint32_t k = blockIdx.x * BS + threadIdx.x; k < nact; k += gridDim.x * BS) {
+        uint32_t i = move[k];
+        double f[6];
+        for (int c = 0; c < 6; ++c) f[c] = s.f[c][i];
+        correct_one(s, i, f, itime, eta, dt_min, dt_max);
+    }
+}
+
+enum : uint32_t { FIN_WRITE_F = 0, FIN_CORRECT = 1 };
+
+// Finish pass of an update: adds the j-chunk partials of each active
+// particle in chunk order (one lane per particle when a cluster has at
+// most 32 chunks, else one workgroup per particle with a fixed-order tree)
+// and either stores the forces (FIN_WRITE_F) or applies the corrector at
+// the cluster's block time ATIME (FIN_CORRECT).
+__global__ __launch_bounds__(BS) void k_finish(Soa s, Ens e, const uint32_t *move,
+                                              const double *partial, uint32_t mode, float eta,
+                                              double dt_min, double dt_max) {
+    __shared__ double red[BS / 64];
+    const unsigned long long total = e.ctl->fin_units;
+    for (unsigned long long u = blockIdx.x; u < total; u += gridDim.x) {
+        const uint32_t c = find_cluster(e.fin_pre, e.K, u);
+        const ClPlan pl = e.plan[c];
+        const uint32_t lu = (uint32_t)(u - e.fin_pre[c]);
+        const uint32_t nact = e.cs[c].nact;
+        const double itime = e.cs[c].atime;
+        const uint32_t *mv = move + e.off[c];
+        const double *part = partial + e.part_pre[c];
+        if (!pl.tree) {
+            uint32_t k = lu * BS + threadIdx.x;
+            if (k < nact) {
+                uint32_t i = mv[k];
+                double f[6];
+                for (int q = 0; q < 6; ++q) {
+                    if (pl.S == 1) {
+                        f[q] = s.f[q][i];
+                    } else {
+                        double acc = 0.0;
+                        for (uint32_t sc = 0; sc < pl.S; ++sc)
+                            acc += part[((size_t)q * pl.S + sc) * pl.rows + k];
+                        f[q] = acc;
+                    }
+                }
+                if (mode == FIN_CORRECT)
+                    correct_one(s, i, f, itime, eta, dt_min, dt_max);
+                if (pl.S > 1)
+                    for (int q = 0; q < 6; ++q) s.f[q][i] = f[q];
+            }
+        } else {
+            uint32_t k = lu;
+            uint32_t i = mv[k];
+            double f[6];
+            for (int q = 0; q < 6; ++q) {
+                double acc = 0.0;
+                for (uint32_t sc = threadIdx.x; sc < pl.S; sc += BS)
+                    acc += part[((size_t)q * pl.S + sc) * pl.rows + k];
+                for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+                __syncthreads();
+                if (__lane_id() == 0) red[threadIdx.x >> 6] = acc;
+                __syncthreads();
+                double t = red[0];
+                for (uint32_t w = 1; w < BS / 64; ++w) t += red[w];
+                f[q] = t;
+            }
+            if (threadIdx.x == 0) {
+                if (mode == FIN_CORRECT) correct_one(s, i, f, itime, eta, dt_min, dt_max);
+                for (int q = 0; q < 6; ++q) s.f[q][i] = f[q];
+            }
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Block-time reductions
+// Block times
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ double block_min(double v, double *red) {
@@ -626,59 +734,188 @@ __device__ __forceinline__ double block_min(double v, double *red) {
     return v;  // valid in thread 0
 }
 
-// Hermite4::next_integration_time (Hermite4.cpp:90-102): min(t_i + dt_i).
-__global__ __launch_bounds__(BS) void k_next_time(const double *t,
-                                                 const double *dt, uint32_t n,
-                                                 DevInfo *info) {
-    __shared__ double red[BS / 64];
-    if (info->done) return;
-    double mn = INFINITY;
-    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < n;
-         i += gridDim.x * BS)
-        mn = fmin(mn, t[i] + dt[i]);
-    mn = block_min(mn, red);
-    if (threadIdx.x == 0) atomicMin(&info->tmin_key, dkey(mn));
+// min(t + dt) of one JT block (padding has dt = +inf).
+__device__ __forceinline__ double block_tmin(const Soa &s, uint32_t b, double *red) {
+    uint32_t i = b * JT + threadIdx.x;
+    double v = s.t[i] + s.dt[i];
+    double m = block_min(v, red);
+    return m;  // valid in thread 0
 }
 
-// Hermite4::init_dt (Hermite4.cpp:108-140) per particle, plus min(dt).
-// Non-positive, infinite or NaN ETA|a|/|j| map to dt_min (the reference's
-// x86 outcome of its undefined (int) cast).
-__global__ __launch_bounds__(BS) void k_init_dt(Soa s, uint32_t n,
-                                               float eta, double itime,
-                                               double dt_min, double dt_max,
-                                               DevInfo *info) {
+// Recomputes the per-block minima of the blocks that had active particles
+// this step (the others did not change).
+__global__ __launch_bounds__(BS) void k_blockmin_dirty(Soa s, Ens e) {
+    __shared__ double red[BS / 64];
+    const uint32_t nd = e.ctl->ndirty;
+    for (uint32_t w = blockIdx.x; w < nd; w += gridDim.x) {
+        uint32_t b = e.dirty[w];
+        __syncthreads();
+        double m = block_tmin(s, b, red);
+        if (threadIdx.x == 0) e.bmin[b] = m;
+    }
+}
+
+__global__ __launch_bounds__(BS) void k_blockmin_all(Soa s, Ens e) {
+    __shared__ double red[BS / 64];
+    for (uint32_t b = blockIdx.x; b < e.nblk; b += gridDim.x) {
+        __syncthreads();
+        double m = block_tmin(s, b, red);
+        if (threadIdx.x == 0) e.bmin[b] = m;
+    }
+}
+
+// Hermite4::next_integration_time (Hermite4.cpp:90-102) per cluster:
+// min(t_i + dt_i) into the cluster's tmin_key (one workgroup per block).
+__global__ __launch_bounds__(BS) void k_next_time(Soa s, Ens e) {
+    __shared__ double red[BS / 64];
+    for (uint32_t b = blockIdx.x; b < e.nblk; b += gridDim.x) {
+        __syncthreads();
+        double m = block_tmin(s, b, red);
+        if (threadIdx.x == 0) atomicMin(&e.cs[cluster_of_block(e, b)].tmin_key, dkey(m));
+    }
+}
+
+// Hermite4::init_dt (Hermite4.cpp:108-140) per particle, plus the cluster
+// minimum of dt in tmin_key.  Non-positive, infinite or NaN ETA|a|/|j|
+// map to dt_min (the reference's x86 outcome of its undefined int cast).
+__global__ __launch_bounds__(BS) void k_init_dt(Soa s, Ens e, float eta, double itime,
+                                               double dt_min, double dt_max) {
 #pragma clang fp contract(off)
     __shared__ double red[BS / 64];
-    double mn = INFINITY;
-    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < n;
-         i += gridDim.x * BS) {
-        double a2 = magnitude(s.f[0][i], s.f[1][i], s.f[2][i]);
-        double j2 = magnitude(s.f[3][i], s.f[4][i], s.f[5][i]);
-        double tmp_dt = eta * (a2 / j2);
-        double d;
-        if (!(tmp_dt > 0.0) || isinf(tmp_dt)) {
-            d = dt_min;
-        } else {
-            int e = (int)(ceil(log(tmp_dt) / log(2.0)) - 1);
-            d = ldexp(1.0, e);
-            if (d < dt_min)
+    for (uint32_t b = blockIdx.x; b < e.nblk; b += gridDim.x) {
+        const uint32_t c = cluster_of_block(e, b);
+        const uint32_t i = b * JT + threadIdx.x;
+        double mn = INFINITY;
+        if (i - e.off[c] < e.cnt[c]) {
+            double a2 = magnitude(s.f[0][i], s.f[1][i], s.f[2][i]);
+            double j2 = magnitude(s.f[3][i], s.f[4][i], s.f[5][i]);
+            double tmp_dt = eta * (a2 / j2);
+            double d;
+            if (!(tmp_dt > 0.0) || isinf(tmp_dt)) {
                 d = dt_min;
-            else if (d > dt_max)
-                d = dt_max;
+            } else {
+                int ex = (int)(ceil(log(tmp_dt) / log(2.0)) - 1);
+                d = ldexp(1.0, ex);
+                if (d < dt_min)
+                    d = dt_min;
+                else if (d > dt_max)
+                    d = dt_max;
+            }
+            s.dt[i] = d;
+            s.t[i] = itime;
+            mn = d;
         }
-        s.dt[i] = d;
-        s.t[i] = itime;
-        mn = fmin(mn, d);
+        __syncthreads();
+        mn = block_min(mn, red);
+        if (threadIdx.x == 0) atomicMin(&e.cs[c].tmin_key, dkey(mn));
     }
-    mn = block_min(mn, red);
-    if (threadIdx.x == 0) atomicMin(&info->tmin_key, dkey(mn));
 }
 
 // ---------------------------------------------------------------------------
-// Energy: sum 1/2 m v^2 - sum_{i<j} m_i m_j / |r_ij| (unsoftened),
-// NbodyUtils::get_energy (NbodyUtils.cpp:427-529).  Pair terms round as
-// there; sums are per-lane Neumaier-compensated, then per workgroup, then
-// over workgroups in index order (deterministic for a fixed grid).
+// Step-loop control
+// ---------------------------------------------------------------------------
+
+// Arms the device step loop of every cluster.  set0: cluster 0 takes
+// (itime0, atime0) first.  A cluster whose ITIME is not < t_end is done
+// at once (the reference's loop test, Hermite4CPU_integration.cpp:81).
+__global__ __launch_bounds__(BS) void k_run_setup(Ens e, double t_end, double t_stop,
+                                                 unsigned long long max_steps, uint32_t set0,
+                                                 double itime0, double atime0) {
+    __shared__ uint32_t cnt[BS / 64];
+    uint32_t nd = 0;
+    for (uint32_t c = threadIdx.x; c < e.K; c += BS) {
+        ClState &cs = e.cs[c];
+        if (set0 && c == 0) {
+            cs.itime = itime0;
+            cs.atime = atime0;
+        }
+        cs.steps = 0;
+        cs.interactions = 0;
+        cs.nsteps = 0;
+        cs.nact = 0;
+        cs.done = (cs.itime < t_end) ? 0u : 1u;
+        nd += cs.done;
+    }
+    for (int o = 32; o > 0; o >>= 1) nd += __shfl_xor(nd, o);
+    if (__lane_id() == 0) cnt[threadIdx.x >> 6] = nd;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t k = 0; k < BS / 64; ++k) t += cnt[k];
+        RunCtl &ctl = *e.ctl;
+        ctl.t_end = t_end;
+        ctl.t_stop = t_stop;
+        ctl.max_steps = max_steps;
+        ctl.ticket = 0;
+        ctl.ndirty = 0;
+        ctl.ndone = t;
+        ctl.items = 0;
+        ctl.fin_units = 0;
+    }
+}
+
+// Loop tail of each live cluster (one workgroup per cluster): ATIME =
+// min over its blocks, counters, history of cluster 0, and the stop tests:
+// output threshold (Hermite4CPU_integration.cpp:102), step budget, and the
+// loop test ITIME < t_end.
+__global__ __launch_bounds__(BS) void k_step_end(Ens e) {
+    __shared__ double red[BS / 64];
+    const uint32_t c = blockIdx.x;
+    ClState *cs = &e.cs[c];
+    // The dirty list of this step has been consumed by k_blockmin_dirty.
+    if (c == 0 && threadIdx.x == 0) e.ctl->ndirty = 0;
+    if (cs->done) return;
+    const uint32_t b0 = e.off[c] / JT;
+    const uint32_t nb = ntiles(e, c);
+    double mn = INFINITY;
+    for (uint32_t b = threadIdx.x; b < nb; b += BS) mn = fmin(mn, e.bmin[b0 + b]);
+    mn = block_min(mn, red);
+    if (threadIdx.x == 0) {
+        const RunCtl &ctl = *e.ctl;
+        double T = cs->atime;
+        uint32_t nact = cs->nact;
+        unsigned long long k = cs->steps;
+        if (c == 0) {
+            e.hist_t[k % e.hist_cap] = T;
+            e.hist_n[k % e.hist_cap] = nact;
+        }
+        cs->itime = T;
+        cs->atime = mn;
+        cs->steps = k + 1;
+        cs->interactions += (unsigned long long)nact * e.cnt[c];
+        cs->nsteps += nact;
+        cs->nact = 0;
+        // (!(mn > T): no particle left ahead of T, the loop cannot advance.)
+        if (T >= ctl.t_stop || cs->steps >= ctl.max_steps || !(T < ctl.t_end) || !(mn > T)) {
+            cs->done = 1;
+            atomicAdd(&e.ctl->ndone, 1u);
+        }
+    }
+}
+
+// Single-cluster ABI helpers.
+__global__ void k_set_nact0(Ens e, uint32_t nact) { e.cs[0].nact = nact; }
+__global__ void k_reset_tmin(Ens e) {
+    for (uint32_t c = 0; c < e.K; ++c) e.cs[c].tmin_key = dkey(INFINITY);
+}
+__global__ void k_reset_select(Ens e) {
+    for (uint32_t c = 0; c < e.K; ++c) e.cs[c].nact = 0;
+    e.ctl->ticket = 0;
+}
+// After init_dt: every cluster at t = itime with ATIME = its minimum dt.
+__global__ void k_set_times_from_tmin(Ens e, double itime) {
+    for (uint32_t c = 0; c < e.K; ++c) {
+        e.cs[c].itime = itime;
+        e.cs[c].atime = dkey_inv(e.cs[c].tmin_key);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Energy: sum 1/2 m v^2 - sum_{i<j} m_i m_j / |r_ij| (unsoftened) of one
+// cluster, NbodyUtils::get_energy (NbodyUtils.cpp:427-529).  Pair terms
+// round as there; sums are per-lane Neumaier-compensated, then per
+// workgroup, then over workgroups in index order (deterministic for a
+// fixed grid).
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void neumaier(double &sum, double &comp, double x) {
@@ -711,10 +948,10 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
     }
 }
 
-// Items are pairs of (i tile, j tile) with j tile >= i tile, numbered row
-// by row; out[4*block] = {epot, epot_comp, ekin, ekin_comp}.
-__global__ __launch_bounds__(BS) void k_energy(Soa s, uint32_t n,
-                                              double *out) {
+// Items are pairs of (i tile, j tile) with j tile >= i tile of the cluster
+// at device offset off, numbered row by row;
+// out[4*block] = {epot, epot_comp, ekin, ekin_comp}.
+__global__ __launch_bounds__(BS) void k_energy(Soa s, uint32_t off, uint32_t n, double *out) {
 #pragma clang fp contract(off)
     __shared__ double4 sr[JT];
     __shared__ double red[2 * (BS / 64)];
@@ -732,18 +969,19 @@ __global__ __launch_bounds__(BS) void k_energy(Soa s, uint32_t n,
         }
         uint32_t jt = it + (uint32_t)rem;
         rem += gridDim.x;
-        uint32_t i = it * BS + threadIdx.x;
-        uint32_t j0 = jt * JT;
+        uint32_t li = it * BS + threadIdx.x;
+        uint32_t lj0 = jt * JT;
         __syncthreads();
         {
-            uint32_t j = j0 + threadIdx.x;
-            if (j < n)
+            uint32_t j = off + lj0 + threadIdx.x;
+            if (lj0 + threadIdx.x < n)
                 sr[threadIdx.x] = make_double4(s.r[0][j], s.r[1][j], s.r[2][j], s.m[j]);
         }
         __syncthreads();
-        if (i < n) {
+        if (li < n) {
+            uint32_t i = off + li;
             double xi = s.r[0][i], yi = s.r[1][i], zi = s.r[2][i], mi = s.m[i];
-            uint32_t jn = min((uint32_t)JT, n - j0);
+            uint32_t jn = min((uint32_t)JT, n - lj0);
             uint32_t jb = (jt == it) ? threadIdx.x + 1 : 0;
             for (uint32_t jj = jb; jj < jn; ++jj) {
                 double4 q = sr[jj];
@@ -778,44 +1016,56 @@ __global__ __launch_bounds__(BS) void k_energy(Soa s, uint32_t n,
 // ---------------------------------------------------------------------------
 
 namespace {
-constexpr uint32_t HIST_CAP = 64;      // per-step history ring (>= RUN_CHUNK)
-constexpr uint32_t RUN_CHUNK = 32;     // block steps per captured graph
-constexpr int N_SOA = 35;              // doubles per particle in `state`
+constexpr uint32_t HIST_CAP = 64;   // per-step history ring of cluster 0 (>= RUN_CHUNK)
+constexpr uint32_t RUN_CHUNK = 32;  // block steps per captured graph
+constexpr int N_SOA = 35;           // doubles per particle in `state`
 }  // namespace
 
 struct h4h_ctx {
-    uint32_t n = 0;
-    uint32_t n_pad = 0;
-    uint32_t n_jt = 0;
+    uint32_t K = 0;       // clusters
+    uint32_t n = 0;       // particles (all clusters)
+    uint32_t n_dev = 0;   // padded device particles
+    uint32_t nblk = 0;    // JT blocks
+    std::vector<uint32_t> off, cnt, hoff;
     double e2 = 0.0;
     float eta = 0.0f;
     double dt_min = 0.0, dt_max = 0.0;
     int precision = H4H_FP64;
     int device = 0;
-    uint32_t grid = DEFAULT_GRID;     // requested persistent workgroups
-    uint32_t grid_eff = DEFAULT_GRID; // used (<= grid, bounded by the work n allows)
+    uint32_t grid = DEFAULT_GRID;      // requested persistent workgroups
     uint32_t max_split = DEFAULT_MAX_SPLIT;
     bool uploaded = false;
     bool timing = false;
-    uint32_t nact_host = 0;  // size of the current device active list
+    bool bm_valid = false;   // per-block minima match t + dt
+    uint32_t nact_host = 0;  // size of cluster 0's active list (single-system calls)
 
     hipStream_t stream = nullptr;
     double *state = nullptr;     // all SoA arrays
-    double *staging = nullptr;   // n_pad * 8 doubles
+    double *staging = nullptr;   // n * 8 doubles
     double *partial = nullptr;   // j-split partials
     size_t partial_len = 0;
     double *eout = nullptr;      // energy per-workgroup sums
     uint32_t *move = nullptr;
-    DevInfo *info = nullptr;
-    DevInfo *info_host = nullptr;  // pinned
+    // cluster tables
+    uint32_t *d_off = nullptr, *d_cnt = nullptr, *d_hoff = nullptr, *d_bcl = nullptr;
+    ClState *d_cs = nullptr;
+    ClPlan *d_plan = nullptr;
+    unsigned long long *d_pre = nullptr;  // items_pre | fin_pre | part_pre
+    RunCtl *d_ctl = nullptr;
+    uint32_t *d_dirty = nullptr;
+    double *d_bmin = nullptr;
     double *hist_t = nullptr;
     uint32_t *hist_n = nullptr;
-    double *hist_t_host = nullptr;   // pinned
-    uint32_t *hist_n_host = nullptr; // pinned
+    // pinned host mirrors
+    RunCtl *ctl_host = nullptr;
+    ClState *cs_host = nullptr;
+    double *hist_t_host = nullptr;
+    uint32_t *hist_n_host = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     Soa s{};
+    Ens e{};
     h4h_stats stats{};
 };
 
@@ -828,12 +1078,22 @@ int check_ctx(h4h_ctx *ctx, bool need_upload = true) {
     return H4H_OK;
 }
 
-uint32_t blocks_for(uint32_t n) { return (n + BS - 1) / BS; }
+int check_single(h4h_ctx *ctx) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (ctx->K != 1)
+        return fail(H4H_EINVAL, "this call acts on one system; use the h4h_ensemble_* calls");
+    return H4H_OK;
+}
+
+uint32_t blocks_for(size_t n) { return (uint32_t)((n + BS - 1) / BS); }
 
 // Largest number of update work items any active-set size can produce for
-// this n (the persistent grid is capped by it, so small systems do not
-// launch thousands of idle workgroups).
-uint32_t max_items(uint32_t n, uint32_t n_jt, uint32_t grid, uint32_t max_split) {
+// one cluster of n particles (the persistent grid of a single-system
+// context is capped by it, so small systems do not launch idle
+// workgroups).
+uint32_t max_items(uint32_t n, uint32_t grid, uint32_t max_split) {
+    uint32_t n_jt = (n + JT - 1) / JT;
     uint32_t best = 1;
     for (uint32_t nact = 1;; nact = (nact < 64 ? nact + 1 : nact * 2)) {
         uint32_t k = nact > n ? n : nact;
@@ -851,10 +1111,15 @@ void drop_graph(h4h_ctx *ctx) {
     ctx->graph = nullptr;
 }
 
-int ensure_partial(h4h_ctx *ctx) {
-    size_t need = (size_t)6 * 2 * ctx->grid * BS;
-    ctx->grid_eff = std::min(ctx->grid, max_items(ctx->n, ctx->n_jt, ctx->grid, ctx->max_split));
+// (Re)sizes the j-split partials for the current grid and updates the
+// launch fields of the cluster table.
+int configure_launch(h4h_ctx *ctx) {
+    uint32_t g = ctx->grid;
+    if (ctx->K == 1) g = std::min(g, max_items(ctx->cnt[0], ctx->grid, ctx->max_split));
+    ctx->e.grid = g;
+    ctx->e.max_split = ctx->max_split;
     drop_graph(ctx);
+    size_t need = (size_t)6 * 2 * BS * ((size_t)g + ctx->K);
     if (ctx->partial && ctx->partial_len >= need) return H4H_OK;
     if (ctx->partial) H4H_HIP(hipFree(ctx->partial));
     ctx->partial = nullptr;
@@ -863,104 +1128,118 @@ int ensure_partial(h4h_ctx *ctx) {
     return H4H_OK;
 }
 
-// The update pass: pair kernel + chunk reduction.  Both read nact and the
-// stop flag from the device.
+uint32_t finish_grid(h4h_ctx *ctx) { return std::min<uint32_t>(1024, std::max<uint32_t>(64, ctx->nblk)); }
+
+// Pair kernel of the current precision over the plan's items.
 int launch_update(h4h_ctx *ctx) {
     if (ctx->precision == H4H_FP64_EXACT) {
-        hipLaunchKernelGGL(k_update_exact, dim3(ctx->grid_eff), dim3(BS), 0, ctx->stream, ctx->s,
-                           ctx->move, ctx->info, ctx->n, ctx->e2);
-        H4H_LAUNCHED();
-        return H4H_OK;
+        hipLaunchKernelGGL(k_update_exact, dim3(ctx->e.grid), dim3(BS), 0, ctx->stream, ctx->s,
+                           ctx->e, ctx->move, ctx->e2);
+    } else {
+        auto *kern = ctx->precision == H4H_MIXED ? k_update_fp64<true> : k_update_fp64<false>;
+        hipLaunchKernelGGL(kern, dim3(ctx->e.grid), dim3(BS), 0, ctx->stream, ctx->s, ctx->e,
+                           ctx->move, ctx->partial, ctx->e2);
     }
-    auto *kern = ctx->precision == H4H_MIXED ? k_update_fp64<true> : k_update_fp64<false>;
-    hipLaunchKernelGGL(kern, dim3(ctx->grid_eff), dim3(BS), 0, ctx->stream, ctx->s, ctx->move,
-                       ctx->info, ctx->partial, ctx->n_jt, ctx->max_split, ctx->e2);
-    H4H_LAUNCHED();
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1024), dim3(BS), 0, ctx->stream, ctx->s, ctx->move,
-                       ctx->info, ctx->partial, ctx->n_jt, ctx->grid_eff, ctx->max_split);
     H4H_LAUNCHED();
     return H4H_OK;
 }
 
-int launch_predict_select(h4h_ctx *ctx, uint32_t flags) {
-    hipLaunchKernelGGL(k_predict_select, dim3(blocks_for(ctx->n)), dim3(BS), 0, ctx->stream,
-                       ctx->s, ctx->n, flags, ctx->move, ctx->info);
+int launch_finish(h4h_ctx *ctx, uint32_t mode) {
+    hipLaunchKernelGGL(k_finish, dim3(finish_grid(ctx)), dim3(BS), 0, ctx->stream, ctx->s, ctx->e,
+                       ctx->move, ctx->partial, mode, ctx->eta, ctx->dt_min, ctx->dt_max);
     H4H_LAUNCHED();
     return H4H_OK;
 }
 
-int launch_correct(h4h_ctx *ctx) {
-    uint32_t grid = std::min<uint32_t>(blocks_for(ctx->n), 1024);
-    hipLaunchKernelGGL(k_correct, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s, ctx->move,
-                       ctx->info, ctx->eta, ctx->dt_min, ctx->dt_max);
+uint32_t exact_flag(h4h_ctx *ctx) { return ctx->precision == H4H_FP64_EXACT ? PS_EXACT : 0u; }
+
+int launch_select(h4h_ctx *ctx, uint32_t flags, double itime) {
+    hipLaunchKernelGGL(k_select, dim3(ctx->nblk), dim3(BS), 0, ctx->stream, ctx->s, ctx->e,
+                       flags | exact_flag(ctx), itime, ctx->move);
     H4H_LAUNCHED();
     return H4H_OK;
 }
 
-int launch_next_time(h4h_ctx *ctx) {
-    uint32_t grid = std::min<uint32_t>(blocks_for(ctx->n), 512);
-    hipLaunchKernelGGL(k_next_time, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s.t, ctx->s.dt,
-                       ctx->n, ctx->info);
+int launch_plan(h4h_ctx *ctx) {
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(BS), 0, ctx->stream, ctx->e,
+                       ctx->precision == H4H_FP64_EXACT ? 1u : 0u);
     H4H_LAUNCHED();
     return H4H_OK;
 }
 
-// One block step on the stream, entirely device-driven: loop head,
-// predictor + selection + save-old, update, corrector, next time, loop tail.
-int enqueue_step(h4h_ctx *ctx) {
+int ensure_blockmin(h4h_ctx *ctx) {
+    if (ctx->bm_valid) return H4H_OK;
+    hipLaunchKernelGGL(k_blockmin_all, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
+                       ctx->stream, ctx->s, ctx->e);
+    H4H_LAUNCHED();
+    ctx->bm_valid = true;
+    return H4H_OK;
+}
+
+// One block step of every live cluster, device-driven: selection with the
+// fused predictor and save-old (its last workgroup plans the update),
+// update, finish (chunk sums + corrector), per-block minima of the touched
+// blocks, loop tail.  Optional HIP events bracket the update kernel.
+int enqueue_step(h4h_ctx *ctx, bool events) {
     int rc;
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
-    H4H_LAUNCHED();
-    if ((rc = launch_predict_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD))) return rc;
+    if ((rc = launch_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD | PS_DEVTIME | PS_DIRTY |
+                                     PS_PLAN,
+                            0.0)))
+        return rc;
+    if (events) H4H_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     if ((rc = launch_update(ctx))) return rc;
-    if ((rc = launch_correct(ctx))) return rc;
-    if ((rc = launch_next_time(ctx))) return rc;
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, ctx->stream, ctx->info, ctx->n,
-                       ctx->hist_t, ctx->hist_n, HIST_CAP);
+    if (events) H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    if ((rc = launch_finish(ctx, FIN_CORRECT))) return rc;
+    hipLaunchKernelGGL(k_blockmin_dirty, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
+                       ctx->stream, ctx->s, ctx->e);
+    H4H_LAUNCHED();
+    hipLaunchKernelGGL(k_step_end, dim3(ctx->K), dim3(BS), 0, ctx->stream, ctx->e);
     H4H_LAUNCHED();
     return H4H_OK;
 }
 
-// Captures RUN_CHUNK steps into one graph (once per context and
-// configuration).  The graph holds kernel nodes only.
+// Captures RUN_CHUNK steps into one graph (once per configuration).  The
+// graph holds kernel nodes only.
 int ensure_graph(h4h_ctx *ctx) {
     if (ctx->graph_exec) return H4H_OK;
     H4H_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     int rc = H4H_OK;
-    for (uint32_t k = 0; k < RUN_CHUNK && rc == H4H_OK; ++k) rc = enqueue_step(ctx);
+    for (uint32_t k = 0; k < RUN_CHUNK && rc == H4H_OK; ++k) rc = enqueue_step(ctx, false);
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    hipError_t err = hipStreamEndCapture(ctx->stream, &g);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
     }
-    if (e != hipSuccess) return fail(H4H_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    if (err != hipSuccess)
+        return fail(H4H_EHIP, std::string("graph capture: ") + hipGetErrorString(err));
     ctx->graph = g;
     H4H_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
     return H4H_OK;
 }
 
-int read_info(h4h_ctx *ctx) {
-    H4H_HIP(hipMemcpyAsync(ctx->info_host, ctx->info, sizeof(DevInfo), hipMemcpyDeviceToHost,
+int read_state(h4h_ctx *ctx) {
+    H4H_HIP(hipMemcpyAsync(ctx->ctl_host, ctx->d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost,
                            ctx->stream));
+    H4H_HIP(hipMemcpyAsync(ctx->cs_host, ctx->d_cs, ctx->K * sizeof(ClState),
+                           hipMemcpyDeviceToHost, ctx->stream));
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     return H4H_OK;
 }
 
-// The update pass with optional HIP-event timing (bench and stats).
-int update_pass(h4h_ctx *ctx, bool sync_for_timing) {
+// Update pass of the single-method calls (plan already on the device).
+int update_pass(h4h_ctx *ctx, uint32_t fin_mode) {
     if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     int rc = launch_update(ctx);
     if (rc) return rc;
+    if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    if ((rc = launch_finish(ctx, fin_mode))) return rc;
+    H4H_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->timing) {
-        H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-        if (sync_for_timing) {
-            H4H_HIP(hipEventSynchronize(ctx->ev[1]));
-            float ms = 0.f;
-            H4H_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-            ctx->stats.update_ms += ms;
-            ctx->stats.update_launches += 1;
-        }
+        float ms = 0.f;
+        H4H_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        ctx->stats.update_ms += ms;
+        ctx->stats.update_launches += 1;
     }
     return H4H_OK;
 }
@@ -968,15 +1247,15 @@ int update_pass(h4h_ctx *ctx, bool sync_for_timing) {
 int upload_cols(h4h_ctx *ctx, const double *src, uint32_t ncols, Cols dst) {
     H4H_HIP(hipMemcpyAsync(ctx->staging, src, (size_t)ctx->n * ncols * sizeof(double),
                            hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_aos_to_soa, dim3(blocks_for(ctx->n)), dim3(BS), 0, ctx->stream,
-                       ctx->staging, ncols, dst, ctx->n);
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(blocks_for(ctx->n_dev)), dim3(BS), 0, ctx->stream,
+                       ctx->staging, ncols, dst, ctx->e);
     H4H_LAUNCHED();
     return H4H_OK;
 }
 
 int download_cols(h4h_ctx *ctx, double *dst, uint32_t ncols, Cols src) {
-    hipLaunchKernelGGL(k_soa_to_aos, dim3(blocks_for(ctx->n)), dim3(BS), 0, ctx->stream, src,
-                       ncols, ctx->staging, ctx->n);
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(blocks_for(ctx->n_dev)), dim3(BS), 0, ctx->stream, src,
+                       ncols, ctx->staging, ctx->e);
     H4H_LAUNCHED();
     H4H_HIP(hipMemcpyAsync(dst, ctx->staging, (size_t)ctx->n * ncols * sizeof(double),
                            hipMemcpyDeviceToHost, ctx->stream));
@@ -984,22 +1263,69 @@ int download_cols(h4h_ctx *ctx, double *dst, uint32_t ncols, Cols src) {
     return H4H_OK;
 }
 
+int upload_scalar(h4h_ctx *ctx, const double *src, double *dst) {
+    H4H_HIP(hipMemcpyAsync(ctx->staging, src, (size_t)ctx->n * sizeof(double),
+                           hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_scalar_to_soa, dim3(blocks_for(ctx->n_dev)), dim3(BS), 0, ctx->stream,
+                       ctx->staging, dst, ctx->e);
+    H4H_LAUNCHED();
+    return H4H_OK;
+}
+
+int download_scalar(h4h_ctx *ctx, const double *src, double *dst) {
+    hipLaunchKernelGGL(k_soa_to_scalar, dim3(blocks_for(ctx->n_dev)), dim3(BS), 0, ctx->stream,
+                       src, ctx->staging, ctx->e);
+    H4H_LAUNCHED();
+    H4H_HIP(hipMemcpyAsync(dst, ctx->staging, (size_t)ctx->n * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    return H4H_OK;
+}
+
 bool valid_precision(int p) { return p == H4H_FP64 || p == H4H_FP64_EXACT || p == H4H_MIXED; }
 
-}  // namespace
+int energy_of(h4h_ctx *ctx, uint32_t c, double *ekin, double *epot) {
+    hipLaunchKernelGGL(k_energy, dim3(ENERGY_GRID), dim3(BS), 0, ctx->stream, ctx->s, ctx->off[c],
+                       ctx->cnt[c], ctx->eout);
+    H4H_LAUNCHED();
+    std::vector<double> h(ENERGY_GRID * 4);
+    H4H_HIP(hipMemcpyAsync(h.data(), ctx->eout, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    // Neumaier sum of the per-workgroup (sum, compensation) pairs.
+    double sp = 0.0, cp = 0.0, sk = 0.0, ck = 0.0;
+    auto add = [](double &s, double &c2, double x) {
+        double t = s + x;
+        if (std::fabs(s) >= std::fabs(x))
+            c2 += (s - t) + x;
+        else
+            c2 += (x - t) + s;
+        s = t;
+    };
+    for (uint32_t b = 0; b < ENERGY_GRID; ++b) {
+        add(sp, cp, h[4 * b + 0]);
+        add(sp, cp, h[4 * b + 1]);
+        add(sk, ck, h[4 * b + 2]);
+        add(sk, ck, h[4 * b + 3]);
+    }
+    if (ekin) *ekin = sk + ck;
+    if (epot) *epot = sp + cp;
+    return H4H_OK;
+}
 
-extern "C" {
-
-int h4h_abi_version(void) { return 101; }
-
-const char *h4h_last_error(void) { return g_err.c_str(); }
-
-int h4h_create(h4h_ctx **out, uint32_t n, double e2, float eta, double dt_min,
-               double dt_max, int precision) {
+int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, float eta,
+                double dt_min, double dt_max, int precision) {
     if (!out) return fail(H4H_EINVAL, "out is null");
     *out = nullptr;
-    if (n < 1) return fail(H4H_EINVAL, "n must be >= 1");
-    if (n > (1u << 30)) return fail(H4H_EINVAL, "n too large");
+    if (K < 1 || !sizes) return fail(H4H_EINVAL, "need at least one cluster");
+    if (K > (1u << 20)) return fail(H4H_EINVAL, "too many clusters");
+    uint64_t total = 0, total_dev = 0;
+    for (uint32_t c = 0; c < K; ++c) {
+        if (sizes[c] < 1) return fail(H4H_EINVAL, "every cluster needs n >= 1");
+        total += sizes[c];
+        total_dev += (sizes[c] + JT - 1) / JT * JT;
+    }
+    if (total_dev > (1u << 30)) return fail(H4H_EINVAL, "n too large");
     if (!(e2 >= 0.0) || !std::isfinite(e2)) return fail(H4H_EINVAL, "e2 must be finite and >= 0");
     if (!(eta > 0.0f) || !std::isfinite(eta)) return fail(H4H_EINVAL, "eta must be > 0");
     if (!(dt_min > 0.0) || !(dt_max >= dt_min) || !std::isfinite(dt_max))
@@ -1011,14 +1337,29 @@ int h4h_create(h4h_ctx **out, uint32_t n, double e2, float eta, double dt_min,
 
     h4h_ctx *ctx = new (std::nothrow) h4h_ctx();
     if (!ctx) return fail(H4H_ENOMEM, "host allocation failed");
-    ctx->n = n;
-    ctx->n_pad = (n + JT - 1) / JT * JT;
-    ctx->n_jt = ctx->n_pad / JT;
+    ctx->K = K;
+    ctx->n = (uint32_t)total;
+    ctx->n_dev = (uint32_t)total_dev;
+    ctx->nblk = ctx->n_dev / JT;
     ctx->e2 = e2;
     ctx->eta = eta;
     ctx->dt_min = dt_min;
     ctx->dt_max = dt_max;
     ctx->precision = precision;
+    ctx->off.resize(K);
+    ctx->cnt.resize(K);
+    ctx->hoff.resize(K);
+    std::vector<uint32_t> bcl(ctx->nblk);
+    uint32_t o = 0, h = 0;
+    for (uint32_t c = 0; c < K; ++c) {
+        ctx->off[c] = o;
+        ctx->cnt[c] = sizes[c];
+        ctx->hoff[c] = h;
+        uint32_t nb = (sizes[c] + JT - 1) / JT;
+        for (uint32_t b = 0; b < nb; ++b) bcl[o / JT + b] = c;
+        o += nb * JT;
+        h += sizes[c];
+    }
 
     auto cleanup = [&](int rc) {
         std::string keep = g_err;
@@ -1029,25 +1370,52 @@ int h4h_create(h4h_ctx **out, uint32_t n, double e2, float eta, double dt_min,
     if (hipGetDevice(&ctx->device) != hipSuccess) return cleanup(fail(H4H_EHIP, "hipGetDevice failed"));
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(H4H_EHIP, "hipStreamCreate failed"));
-    size_t np = ctx->n_pad;
-    if (hipMalloc(&ctx->state, np * N_SOA * sizeof(double)) != hipSuccess)
-        return cleanup(fail(H4H_ENOMEM, "device allocation of the state failed"));
-    if (hipMalloc(&ctx->staging, np * 8 * sizeof(double)) != hipSuccess)
-        return cleanup(fail(H4H_ENOMEM, "device allocation of staging failed"));
-    if (hipMalloc(&ctx->eout, ENERGY_GRID * 4 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&ctx->move, np * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&ctx->info, sizeof(DevInfo)) != hipSuccess ||
-        hipMalloc(&ctx->hist_t, HIST_CAP * sizeof(double)) != hipSuccess ||
-        hipMalloc(&ctx->hist_n, HIST_CAP * sizeof(uint32_t)) != hipSuccess)
-        return cleanup(fail(H4H_ENOMEM, "device allocation failed"));
-    if (hipHostMalloc(&ctx->info_host, sizeof(DevInfo), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&ctx->hist_t_host, HIST_CAP * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&ctx->hist_n_host, HIST_CAP * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
-        return cleanup(fail(H4H_ENOMEM, "pinned allocation failed"));
+    size_t np = ctx->n_dev;
+    bool ok = hipMalloc(&ctx->state, np * N_SOA * sizeof(double)) == hipSuccess &&
+              hipMalloc(&ctx->staging, (size_t)ctx->n * 8 * sizeof(double)) == hipSuccess &&
+              hipMalloc(&ctx->eout, ENERGY_GRID * 4 * sizeof(double)) == hipSuccess &&
+              hipMalloc(&ctx->move, np * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&ctx->d_off, K * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&ctx->d_cnt, K * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&ctx->d_hoff, K * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&ctx->d_bcl, ctx->nblk * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&ctx->d_cs, K * sizeof(ClState)) == hipSuccess &&
+              hipMalloc(&ctx->d_plan, K * sizeof(ClPlan)) == hipSuccess &&
+              hipMalloc(&ctx->d_pre, 3 * (size_t)(K + 1) * sizeof(unsigned long long)) == hipSuccess &&
+              hipMalloc(&ctx->d_ctl, sizeof(RunCtl)) == hipSuccess &&
+              hipMalloc(&ctx->d_dirty, ctx->nblk * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&ctx->d_bmin, ctx->nblk * sizeof(double)) == hipSuccess &&
+              hipMalloc(&ctx->hist_t, HIST_CAP * sizeof(double)) == hipSuccess &&
+              hipMalloc(&ctx->hist_n, HIST_CAP * sizeof(uint32_t)) == hipSuccess;
+    if (!ok) return cleanup(fail(H4H_ENOMEM, "device allocation failed"));
+    ok = hipHostMalloc(&ctx->ctl_host, sizeof(RunCtl), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc(&ctx->cs_host, K * sizeof(ClState), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc(&ctx->hist_t_host, HIST_CAP * sizeof(double), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc(&ctx->hist_n_host, HIST_CAP * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
+    if (!ok) return cleanup(fail(H4H_ENOMEM, "pinned allocation failed"));
     for (int k = 0; k < 4; ++k)
         if (hipEventCreate(&ctx->ev[k]) != hipSuccess)
             return cleanup(fail(H4H_EHIP, "hipEventCreate failed"));
-    int rc = ensure_partial(ctx);
+
+    Ens &e = ctx->e;
+    e.K = K;
+    e.nblk = ctx->nblk;
+    e.off = ctx->d_off;
+    e.cnt = ctx->d_cnt;
+    e.hoff = ctx->d_hoff;
+    e.bcl = ctx->d_bcl;
+    e.cs = ctx->d_cs;
+    e.plan = ctx->d_plan;
+    e.items_pre = ctx->d_pre;
+    e.fin_pre = ctx->d_pre + (K + 1);
+    e.part_pre = ctx->d_pre + 2 * (size_t)(K + 1);
+    e.ctl = ctx->d_ctl;
+    e.dirty = ctx->d_dirty;
+    e.bmin = ctx->d_bmin;
+    e.hist_t = ctx->hist_t;
+    e.hist_n = ctx->hist_n;
+    e.hist_cap = HIST_CAP;
+    int rc = configure_launch(ctx);
     if (rc) return cleanup(rc);
 
     double *b = ctx->state;
@@ -1068,13 +1436,40 @@ int h4h_create(h4h_ctx **out, uint32_t n, double e2, float eta, double dt_min,
     s.dt = take();
     for (int c = 0; c < 6; ++c) s.p[c] = take();
     s.pm = take();
-    if (hipMemsetAsync(ctx->state, 0, np * N_SOA * sizeof(double), ctx->stream) != hipSuccess ||
-        hipMemsetAsync(ctx->info, 0, sizeof(DevInfo), ctx->stream) != hipSuccess)
-        return cleanup(fail(H4H_EHIP, "hipMemset failed"));
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess)
-        return cleanup(fail(H4H_EHIP, "initialisation failed"));
+    ok = hipMemcpyAsync(ctx->d_off, ctx->off.data(), K * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+         hipMemcpyAsync(ctx->d_cnt, ctx->cnt.data(), K * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+         hipMemcpyAsync(ctx->d_hoff, ctx->hoff.data(), K * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+         hipMemcpyAsync(ctx->d_bcl, bcl.data(), ctx->nblk * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->state, 0, np * N_SOA * sizeof(double), ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->d_cs, 0, K * sizeof(ClState), ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->d_ctl, 0, sizeof(RunCtl), ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->d_pre, 0, 3 * (size_t)(K + 1) * sizeof(unsigned long long), ctx->stream) == hipSuccess &&
+         hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (!ok) return cleanup(fail(H4H_EHIP, "initialisation failed"));
     *out = ctx;
     return H4H_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int h4h_abi_version(void) { return 102; }
+
+const char *h4h_last_error(void) { return g_err.c_str(); }
+
+int h4h_create(h4h_ctx **out, uint32_t n, double e2, float eta, double dt_min, double dt_max,
+               int precision) {
+    if (n < 1) {
+        if (out) *out = nullptr;
+        return fail(H4H_EINVAL, "n must be >= 1");
+    }
+    return create_impl(out, 1, &n, e2, eta, dt_min, dt_max, precision);
+}
+
+int h4h_create_ensemble(h4h_ctx **out, uint32_t nclusters, const uint32_t *sizes, double e2,
+                        float eta, double dt_min, double dt_max, int precision) {
+    return create_impl(out, nclusters, sizes, e2, eta, dt_min, dt_max, precision);
 }
 
 int h4h_destroy(h4h_ctx *ctx) {
@@ -1083,17 +1478,14 @@ int h4h_destroy(h4h_ctx *ctx) {
     drop_graph(ctx);
     for (int k = 0; k < 4; ++k)
         if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
-    if (ctx->state) (void)hipFree(ctx->state);
-    if (ctx->staging) (void)hipFree(ctx->staging);
-    if (ctx->partial) (void)hipFree(ctx->partial);
-    if (ctx->eout) (void)hipFree(ctx->eout);
-    if (ctx->move) (void)hipFree(ctx->move);
-    if (ctx->info) (void)hipFree(ctx->info);
-    if (ctx->hist_t) (void)hipFree(ctx->hist_t);
-    if (ctx->hist_n) (void)hipFree(ctx->hist_n);
-    if (ctx->info_host) (void)hipHostFree(ctx->info_host);
-    if (ctx->hist_t_host) (void)hipHostFree(ctx->hist_t_host);
-    if (ctx->hist_n_host) (void)hipHostFree(ctx->hist_n_host);
+    void *dev[] = {ctx->state, ctx->staging, ctx->partial, ctx->eout, ctx->move, ctx->d_off,
+                   ctx->d_cnt, ctx->d_hoff, ctx->d_bcl, ctx->d_cs, ctx->d_plan, ctx->d_pre,
+                   ctx->d_ctl, ctx->d_dirty, ctx->d_bmin, ctx->hist_t, ctx->hist_n};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    void *host[] = {ctx->ctl_host, ctx->cs_host, ctx->hist_t_host, ctx->hist_n_host};
+    for (void *p : host)
+        if (p) (void)hipHostFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return H4H_OK;
@@ -1114,7 +1506,7 @@ int h4h_set_launch(h4h_ctx *ctx, uint32_t workgroups, uint32_t max_jsplit) {
     if (workgroups > 65536) return fail(H4H_EINVAL, "workgroups > 65536");
     ctx->grid = workgroups ? workgroups : DEFAULT_GRID;
     ctx->max_split = max_jsplit ? max_jsplit : DEFAULT_MAX_SPLIT;
-    return ensure_partial(ctx);
+    return configure_launch(ctx);
 }
 
 int h4h_upload(h4h_ctx *ctx, const double *r4, const double *v4, const double *f6,
@@ -1123,9 +1515,8 @@ int h4h_upload(h4h_ctx *ctx, const double *r4, const double *v4, const double *f
     if (rc) return rc;
     if (!r4 || !v4) return fail(H4H_EINVAL, "r4 and v4 are required");
     const Soa &s = ctx->s;
-    size_t nb = (size_t)ctx->n_pad * sizeof(double);
     // Zero everything first (init_data semantics), then fill.
-    H4H_HIP(hipMemsetAsync(ctx->state, 0, nb * N_SOA, ctx->stream));
+    H4H_HIP(hipMemsetAsync(ctx->state, 0, (size_t)ctx->n_dev * N_SOA * sizeof(double), ctx->stream));
     Cols c{};
     c.c[0] = s.r[0]; c.c[1] = s.r[1]; c.c[2] = s.r[2]; c.c[3] = s.m;
     if ((rc = upload_cols(ctx, r4, 4, c))) return rc;
@@ -1137,15 +1528,15 @@ int h4h_upload(h4h_ctx *ctx, const double *r4, const double *v4, const double *f
         for (int k = 0; k < 6; ++k) cf.c[k] = s.f[k];
         if ((rc = upload_cols(ctx, f6, 6, cf))) return rc;
     }
-    if (t) H4H_HIP(hipMemcpyAsync(s.t, t, ctx->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    if (dt) H4H_HIP(hipMemcpyAsync(s.dt, dt, ctx->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_init_predictor, dim3(blocks_for(ctx->n_pad)), dim3(BS), 0, ctx->stream,
-                       ctx->s, ctx->n, ctx->n_pad);
+    if (t && (rc = upload_scalar(ctx, t, s.t))) return rc;
+    if (dt && (rc = upload_scalar(ctx, dt, s.dt))) return rc;
+    hipLaunchKernelGGL(k_init_predictor, dim3(blocks_for(ctx->n_dev)), dim3(BS), 0, ctx->stream,
+                       ctx->s, ctx->e);
     H4H_LAUNCHED();
-    hipLaunchKernelGGL(k_set_nact, dim3(1), dim3(1), 0, ctx->stream, ctx->info, 0u);
-    H4H_LAUNCHED();
+    H4H_HIP(hipMemsetAsync(ctx->d_cs, 0, ctx->K * sizeof(ClState), ctx->stream));
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     ctx->nact_host = 0;
+    ctx->bm_valid = false;
     ctx->uploaded = true;
     return H4H_OK;
 }
@@ -1185,9 +1576,8 @@ int h4h_download(h4h_ctx *ctx, double *r4, double *v4, double *f6, double *t, do
         for (int k = 0; k < 3; ++k) c.c[k] = s.a3[k];
         if ((rc = download_cols(ctx, a3, 3, c))) return rc;
     }
-    if (t) H4H_HIP(hipMemcpyAsync(t, s.t, ctx->n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    if (dt) H4H_HIP(hipMemcpyAsync(dt, s.dt, ctx->n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    if (t && (rc = download_scalar(ctx, s.t, t))) return rc;
+    if (dt && (rc = download_scalar(ctx, s.dt, dt))) return rc;
     return H4H_OK;
 }
 
@@ -1202,9 +1592,7 @@ int h4h_download_predictor(h4h_ctx *ctx, double *p6, float *pm) {
     }
     if (pm) {
         std::vector<double> tmp(ctx->n);
-        H4H_HIP(hipMemcpyAsync(tmp.data(), s.pm, ctx->n * sizeof(double), hipMemcpyDeviceToHost,
-                               ctx->stream));
-        H4H_HIP(hipStreamSynchronize(ctx->stream));
+        if ((rc = download_scalar(ctx, s.pm, tmp.data()))) return rc;
         for (uint32_t i = 0; i < ctx->n; ++i) pm[i] = (float)tmp[i];
     }
     return H4H_OK;
@@ -1222,16 +1610,14 @@ int h4h_upload_predictor(h4h_ctx *ctx, const double *p6, const float *pm) {
     if (pm) {
         std::vector<double> tmp(ctx->n);
         for (uint32_t i = 0; i < ctx->n; ++i) tmp[i] = (double)pm[i];
-        H4H_HIP(hipMemcpyAsync(s.pm, tmp.data(), ctx->n * sizeof(double), hipMemcpyHostToDevice,
-                               ctx->stream));
-        H4H_HIP(hipStreamSynchronize(ctx->stream));
+        if ((rc = upload_scalar(ctx, tmp.data(), s.pm))) return rc;
     }
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     return H4H_OK;
 }
 
 int h4h_set_active(h4h_ctx *ctx, const uint32_t *move, uint32_t nact) {
-    int rc = check_ctx(ctx);
+    int rc = check_single(ctx);
     if (rc) return rc;
     if (nact > ctx->n) return fail(H4H_EINVAL, "nact > n");
     if (nact && !move) return fail(H4H_EINVAL, "move is null");
@@ -1240,15 +1626,16 @@ int h4h_set_active(h4h_ctx *ctx, const uint32_t *move, uint32_t nact) {
     if (nact)
         H4H_HIP(hipMemcpyAsync(ctx->move, move, nact * sizeof(uint32_t), hipMemcpyHostToDevice,
                                ctx->stream));
-    hipLaunchKernelGGL(k_set_nact, dim3(1), dim3(1), 0, ctx->stream, ctx->info, nact);
+    hipLaunchKernelGGL(k_set_nact0, dim3(1), dim3(1), 0, ctx->stream, ctx->e, nact);
     H4H_LAUNCHED();
+    if ((rc = launch_plan(ctx))) return rc;
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     ctx->nact_host = nact;
     return H4H_OK;
 }
 
 int h4h_get_active(h4h_ctx *ctx, uint32_t *move, uint32_t *nact) {
-    int rc = check_ctx(ctx);
+    int rc = check_single(ctx);
     if (rc) return rc;
     if (!nact) return fail(H4H_EINVAL, "nact is null");
     *nact = ctx->nact_host;
@@ -1261,131 +1648,125 @@ int h4h_get_active(h4h_ctx *ctx, uint32_t *move, uint32_t *nact) {
     return H4H_OK;
 }
 
+// Initial forces of every cluster: each active list = the whole cluster.
 int h4h_init_acc_jrk(h4h_ctx *ctx) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_identity_active, dim3(blocks_for(ctx->n)), dim3(BS), 0, ctx->stream,
-                       ctx->move, ctx->n);
+    hipLaunchKernelGGL(k_identity_active, dim3(blocks_for(ctx->n_dev)), dim3(BS), 0, ctx->stream,
+                       ctx->move, ctx->e);
     H4H_LAUNCHED();
-    hipLaunchKernelGGL(k_set_nact, dim3(1), dim3(1), 0, ctx->stream, ctx->info, ctx->n);
-    H4H_LAUNCHED();
-    ctx->nact_host = ctx->n;
-    if ((rc = update_pass(ctx, true))) return rc;
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    if ((rc = launch_plan(ctx))) return rc;
+    if ((rc = update_pass(ctx, FIN_WRITE_F))) return rc;
+    ctx->nact_host = ctx->K == 1 ? ctx->cnt[0] : 0;
     return H4H_OK;
 }
 
+static int init_dt_impl(h4h_ctx *ctx, float eta, double itime) {
+    hipLaunchKernelGGL(k_reset_tmin, dim3(1), dim3(1), 0, ctx->stream, ctx->e);
+    H4H_LAUNCHED();
+    hipLaunchKernelGGL(k_init_dt, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
+                       ctx->stream, ctx->s, ctx->e, eta, itime, ctx->dt_min, ctx->dt_max);
+    H4H_LAUNCHED();
+    hipLaunchKernelGGL(k_set_times_from_tmin, dim3(1), dim3(1), 0, ctx->stream, ctx->e, itime);
+    H4H_LAUNCHED();
+    ctx->bm_valid = false;
+    return read_state(ctx);
+}
+
 int h4h_init_dt(h4h_ctx *ctx, double *ctime, float eta, double itime) {
-    int rc = check_ctx(ctx);
+    int rc = check_single(ctx);
     if (rc) return rc;
     if (!ctime) return fail(H4H_EINVAL, "ctime is null");
-    hipLaunchKernelGGL(k_reset_tmin, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
-    H4H_LAUNCHED();
-    uint32_t grid = std::min<uint32_t>(blocks_for(ctx->n), 1024);
-    hipLaunchKernelGGL(k_init_dt, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s, ctx->n, eta, itime,
-                       ctx->dt_min, ctx->dt_max, ctx->info);
-    H4H_LAUNCHED();
-    if ((rc = read_info(ctx))) return rc;
-    double mn = dkey_inv(ctx->info_host->tmin_key);
+    if ((rc = init_dt_impl(ctx, eta, itime))) return rc;
+    double mn = dkey_inv(ctx->cs_host[0].tmin_key);
     if (mn < *ctime) *ctime = mn;
     return H4H_OK;
 }
 
 int h4h_find_particles_to_move(h4h_ctx *ctx, double itime, uint32_t *nact) {
-    int rc = check_ctx(ctx);
+    int rc = check_single(ctx);
     if (rc) return rc;
     if (!nact) return fail(H4H_EINVAL, "nact is null");
-    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, ctx->stream, ctx->info, itime);
+    hipLaunchKernelGGL(k_reset_select, dim3(1), dim3(1), 0, ctx->stream, ctx->e);
     H4H_LAUNCHED();
-    hipLaunchKernelGGL(k_begin_select, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
-    H4H_LAUNCHED();
-    if ((rc = launch_predict_select(ctx, PS_SELECT))) return rc;
-    if ((rc = read_info(ctx))) return rc;
-    ctx->nact_host = ctx->info_host->nact;
+    if ((rc = launch_select(ctx, PS_SELECT | PS_PLAN, itime))) return rc;
+    if ((rc = read_state(ctx))) return rc;
+    ctx->nact_host = ctx->cs_host[0].nact;
     *nact = ctx->nact_host;
     return H4H_OK;
 }
 
 int h4h_save_old_acc_jrk(h4h_ctx *ctx) {
-    int rc = check_ctx(ctx);
+    int rc = check_single(ctx);
     if (rc) return rc;
     if (!ctx->nact_host) return H4H_OK;
     uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(blocks_for(ctx->nact_host), 1024));
-    hipLaunchKernelGGL(k_save_old, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s, ctx->move,
-                       ctx->info);
+    hipLaunchKernelGGL(k_save_old, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s, ctx->e, ctx->move);
     H4H_LAUNCHED();
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     return H4H_OK;
 }
 
 int h4h_predicted_pos_vel(h4h_ctx *ctx, double itime) {
-    int rc = check_ctx(ctx);
+    int rc = check_single(ctx);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, ctx->stream, ctx->info, itime);
-    H4H_LAUNCHED();
-    if ((rc = launch_predict_select(ctx, PS_PREDICT))) return rc;
+    if ((rc = launch_select(ctx, PS_PREDICT, itime))) return rc;
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     return H4H_OK;
 }
 
 int h4h_update_acc_jrk(h4h_ctx *ctx, uint32_t nact) {
-    int rc = check_ctx(ctx);
+    int rc = check_single(ctx);
     if (rc) return rc;
     if (nact != ctx->nact_host)
         return fail(H4H_EINVAL, "nact differs from the current active list size");
-    if (nact) {
-        if ((rc = update_pass(ctx, true))) return rc;
-    }
+    if (nact && (rc = update_pass(ctx, FIN_WRITE_F))) return rc;
     ctx->stats.block_steps += 1;
     ctx->stats.interactions += (uint64_t)nact * ctx->n;
     ctx->stats.nsteps += nact;
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
     return H4H_OK;
 }
 
 int h4h_correction_pos_vel(h4h_ctx *ctx, double itime, uint32_t nact) {
-    int rc = check_ctx(ctx);
+    int rc = check_single(ctx);
     if (rc) return rc;
     if (nact != ctx->nact_host)
         return fail(H4H_EINVAL, "nact differs from the current active list size");
-    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, ctx->stream, ctx->info, itime);
+    uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(blocks_for(nact), 1024));
+    hipLaunchKernelGGL(k_correct, dim3(grid), dim3(BS), 0, ctx->stream, ctx->s, ctx->e, ctx->move,
+                       itime, ctx->eta, ctx->dt_min, ctx->dt_max);
     H4H_LAUNCHED();
-    if ((rc = launch_correct(ctx))) return rc;
     H4H_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->bm_valid = false;
     return H4H_OK;
 }
 
 int h4h_next_integration_time(h4h_ctx *ctx, double *atime) {
-    int rc = check_ctx(ctx);
+    int rc = check_single(ctx);
     if (rc) return rc;
     if (!atime) return fail(H4H_EINVAL, "atime is null");
-    hipLaunchKernelGGL(k_reset_tmin, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
+    hipLaunchKernelGGL(k_reset_tmin, dim3(1), dim3(1), 0, ctx->stream, ctx->e);
     H4H_LAUNCHED();
-    if ((rc = launch_next_time(ctx))) return rc;
-    if ((rc = read_info(ctx))) return rc;
-    *atime = dkey_inv(ctx->info_host->tmin_key);
+    hipLaunchKernelGGL(k_next_time, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
+                       ctx->stream, ctx->s, ctx->e);
+    H4H_LAUNCHED();
+    if ((rc = read_state(ctx))) return rc;
+    *atime = dkey_inv(ctx->cs_host[0].tmin_key);
     return H4H_OK;
 }
 
 int h4h_step(h4h_ctx *ctx, double itime, uint32_t *nact, double *atime) {
-    int rc = check_ctx(ctx);
+    int rc = check_single(ctx);
     if (rc) return rc;
+    if ((rc = ensure_blockmin(ctx))) return rc;
     if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-    // One-step loop: ITIME = ATIME = itime, stop after the step.
-    hipLaunchKernelGGL(k_run_setup, dim3(1), dim3(1), 0, ctx->stream, ctx->info, -INFINITY, itime,
-                       (double)INFINITY, -INFINITY, 1ull);
+    // One-step loop: ATIME = itime, stop after the step.
+    hipLaunchKernelGGL(k_run_setup, dim3(1), dim3(BS), 0, ctx->stream, ctx->e, (double)INFINITY,
+                       -INFINITY, 1ull, 1u, -INFINITY, itime);
     H4H_LAUNCHED();
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->info);
-    H4H_LAUNCHED();
-    if ((rc = launch_predict_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD))) return rc;
-    if ((rc = update_pass(ctx, false))) return rc;
-    if ((rc = launch_correct(ctx))) return rc;
-    if ((rc = launch_next_time(ctx))) return rc;
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, ctx->stream, ctx->info, ctx->n,
-                       ctx->hist_t, ctx->hist_n, HIST_CAP);
-    H4H_LAUNCHED();
+    if ((rc = enqueue_step(ctx, ctx->timing))) return rc;
     if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
-    if ((rc = read_info(ctx))) return rc;
+    if ((rc = read_state(ctx))) return rc;
     if (ctx->timing) {
         float ms = 0.f;
         H4H_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
@@ -1394,45 +1775,47 @@ int h4h_step(h4h_ctx *ctx, double itime, uint32_t *nact, double *atime) {
         ctx->stats.update_ms += ms;
         ctx->stats.update_launches += 1;
     }
-    ctx->nact_host = ctx->info_host->nact;
+    const ClState &cs = ctx->cs_host[0];
+    uint32_t k = (uint32_t)cs.nsteps;  // the step's nact (counters were reset)
+    ctx->nact_host = 0;
     ctx->stats.block_steps += 1;
-    ctx->stats.interactions += (uint64_t)ctx->nact_host * ctx->n;
-    ctx->stats.nsteps += ctx->nact_host;
-    if (nact) *nact = ctx->nact_host;
-    if (atime) *atime = ctx->info_host->atime;
+    ctx->stats.interactions += cs.interactions;
+    ctx->stats.nsteps += k;
+    if (nact) *nact = k;
+    if (atime) *atime = cs.atime;
     return H4H_OK;
 }
 
-int h4h_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_steps, h4h_run_info *io,
-            double *step_time, uint32_t *step_nact) {
-    int rc = check_ctx(ctx);
+// Replays captured chunks of RUN_CHUNK steps until every cluster's loop has
+// stopped; copies cluster 0's per-step history when asked.
+static int run_chunks(h4h_ctx *ctx, double *step_time, uint32_t *step_nact, uint64_t cap) {
+    int rc = ensure_graph(ctx);
     if (rc) return rc;
-    if (!io) return fail(H4H_EINVAL, "io is null");
-    if (max_steps == 0) return fail(H4H_EINVAL, "max_steps must be >= 1");
-    if ((rc = ensure_graph(ctx))) return rc;
-    hipLaunchKernelGGL(k_run_setup, dim3(1), dim3(1), 0, ctx->stream, ctx->info, io->itime,
-                       io->atime, t_end, t_stop, (unsigned long long)max_steps);
-    H4H_LAUNCHED();
     uint64_t seen = 0;
+    bool hist = (step_time || step_nact) && ctx->K == 1;
     if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     for (;;) {
         H4H_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
-        H4H_HIP(hipMemcpyAsync(ctx->info_host, ctx->info, sizeof(DevInfo), hipMemcpyDeviceToHost,
+        H4H_HIP(hipMemcpyAsync(ctx->ctl_host, ctx->d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost,
                                ctx->stream));
-        if (step_time || step_nact) {
+        H4H_HIP(hipMemcpyAsync(ctx->cs_host, ctx->d_cs, sizeof(ClState), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        if (hist) {
             H4H_HIP(hipMemcpyAsync(ctx->hist_t_host, ctx->hist_t, HIST_CAP * sizeof(double),
                                    hipMemcpyDeviceToHost, ctx->stream));
             H4H_HIP(hipMemcpyAsync(ctx->hist_n_host, ctx->hist_n, HIST_CAP * sizeof(uint32_t),
                                    hipMemcpyDeviceToHost, ctx->stream));
         }
         H4H_HIP(hipStreamSynchronize(ctx->stream));
-        const DevInfo &d = *ctx->info_host;
-        for (uint64_t k = seen; k < d.steps; ++k) {
-            if (step_time) step_time[k] = ctx->hist_t_host[k % HIST_CAP];
-            if (step_nact) step_nact[k] = ctx->hist_n_host[k % HIST_CAP];
+        if (hist) {
+            uint64_t steps = ctx->cs_host[0].steps;
+            for (uint64_t k = seen; k < steps && k < cap; ++k) {
+                if (step_time) step_time[k] = ctx->hist_t_host[k % HIST_CAP];
+                if (step_nact) step_nact[k] = ctx->hist_n_host[k % HIST_CAP];
+            }
+            seen = steps;
         }
-        seen = d.steps;
-        if (d.done) break;
+        if (ctx->ctl_host->ndone >= ctx->K) break;
     }
     if (ctx->timing) {
         H4H_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
@@ -1441,20 +1824,42 @@ int h4h_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_steps, h4h_r
         H4H_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
         ctx->stats.step_ms += ms;
     }
-    const DevInfo &d = *ctx->info_host;
+    return read_state(ctx);
+}
+
+int h4h_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_steps, h4h_run_info *io,
+            double *step_time, uint32_t *step_nact) {
+    int rc = check_single(ctx);
+    if (rc) return rc;
+    if (!io) return fail(H4H_EINVAL, "io is null");
+    if (max_steps == 0) return fail(H4H_EINVAL, "max_steps must be >= 1");
+    if ((rc = ensure_blockmin(ctx))) return rc;
+    hipLaunchKernelGGL(k_run_setup, dim3(1), dim3(BS), 0, ctx->stream, ctx->e, t_end, t_stop,
+                       (unsigned long long)max_steps, 1u, io->itime, io->atime);
+    H4H_LAUNCHED();
+    if ((rc = run_chunks(ctx, step_time, step_nact, max_steps))) return rc;
+    const ClState &d = ctx->cs_host[0];
     io->itime = d.itime;
     io->atime = d.atime;
     io->steps = d.steps;
     io->interactions = d.interactions;
     io->nsteps = d.nsteps;
-    io->last_nact = d.steps ? d.nact : 0;
+    uint32_t last = 0;
+    if (d.steps) {
+        // nact of the last step: from the history ring
+        H4H_HIP(hipMemcpyAsync(ctx->hist_n_host, ctx->hist_n, HIST_CAP * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        H4H_HIP(hipStreamSynchronize(ctx->stream));
+        last = ctx->hist_n_host[(d.steps - 1) % HIST_CAP];
+    }
+    io->last_nact = last;
     if (d.steps && d.itime >= t_stop)
         io->stopped = H4H_STOP_OUTPUT;
     else if (d.steps >= max_steps)
         io->stopped = H4H_STOP_MAXSTEPS;
     else
         io->stopped = H4H_STOP_END;
-    ctx->nact_host = io->last_nact;
+    ctx->nact_host = 0;
     ctx->stats.block_steps += d.steps;
     ctx->stats.interactions += d.interactions;
     ctx->stats.nsteps += d.nsteps;
@@ -1462,33 +1867,66 @@ int h4h_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_steps, h4h_r
 }
 
 int h4h_get_energy(h4h_ctx *ctx, double *ekin, double *epot) {
+    int rc = check_single(ctx);
+    if (rc) return rc;
+    return energy_of(ctx, 0, ekin, epot);
+}
+
+int h4h_ensemble_init(h4h_ctx *ctx, float eta_s) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_energy, dim3(ENERGY_GRID), dim3(BS), 0, ctx->stream, ctx->s, ctx->n,
-                       ctx->eout);
+    if ((rc = h4h_init_acc_jrk(ctx))) return rc;
+    return init_dt_impl(ctx, eta_s, 0.0);
+}
+
+int h4h_ensemble_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_steps,
+                     h4h_cluster_info *info) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (max_steps == 0) return fail(H4H_EINVAL, "max_steps must be >= 1");
+    if ((rc = ensure_blockmin(ctx))) return rc;
+    hipLaunchKernelGGL(k_run_setup, dim3(1), dim3(BS), 0, ctx->stream, ctx->e, t_end, t_stop,
+                       (unsigned long long)max_steps, 0u, 0.0, 0.0);
     H4H_LAUNCHED();
-    std::vector<double> h(ENERGY_GRID * 4);
-    H4H_HIP(hipMemcpyAsync(h.data(), ctx->eout, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                           ctx->stream));
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
-    // Neumaier sum of the per-workgroup (sum, compensation) pairs.
-    double sp = 0.0, cp = 0.0, sk = 0.0, ck = 0.0;
-    auto add = [](double &s, double &c, double x) {
-        double t = s + x;
-        if (std::fabs(s) >= std::fabs(x))
-            c += (s - t) + x;
-        else
-            c += (x - t) + s;
-        s = t;
-    };
-    for (uint32_t b = 0; b < ENERGY_GRID; ++b) {
-        add(sp, cp, h[4 * b + 0]);
-        add(sp, cp, h[4 * b + 1]);
-        add(sk, ck, h[4 * b + 2]);
-        add(sk, ck, h[4 * b + 3]);
+    if ((rc = run_chunks(ctx, nullptr, nullptr, 0))) return rc;
+    for (uint32_t c = 0; c < ctx->K; ++c) {
+        const ClState &d = ctx->cs_host[c];
+        ctx->stats.block_steps += d.steps;
+        ctx->stats.interactions += d.interactions;
+        ctx->stats.nsteps += d.nsteps;
+        if (info) {
+            info[c].itime = d.itime;
+            info[c].atime = d.atime;
+            info[c].steps = d.steps;
+            info[c].interactions = d.interactions;
+            info[c].nsteps = d.nsteps;
+            info[c].stopped = d.steps && d.itime >= t_stop ? H4H_STOP_OUTPUT
+                              : d.steps >= max_steps    ? H4H_STOP_MAXSTEPS
+                                                        : H4H_STOP_END;
+        }
     }
-    if (ekin) *ekin = sk + ck;
-    if (epot) *epot = sp + cp;
+    ctx->nact_host = 0;
+    return H4H_OK;
+}
+
+int h4h_ensemble_energy(h4h_ctx *ctx, double *ekin, double *epot) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    for (uint32_t c = 0; c < ctx->K; ++c) {
+        double k = 0.0, p = 0.0;
+        if ((rc = energy_of(ctx, c, &k, &p))) return rc;
+        if (ekin) ekin[c] = k;
+        if (epot) epot[c] = p;
+    }
+    return H4H_OK;
+}
+
+int h4h_ensemble_sizes(h4h_ctx *ctx, uint32_t *nclusters, uint32_t *sizes) {
+    int rc = check_ctx(ctx, false);
+    if (rc) return rc;
+    if (nclusters) *nclusters = ctx->K;
+    if (sizes)
+        for (uint32_t c = 0; c < ctx->K; ++c) sizes[c] = ctx->cnt[c];
     return H4H_OK;
 }
 

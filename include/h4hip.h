@@ -203,6 +203,47 @@ typedef struct h4h_run_info {
 int h4h_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_steps,
             h4h_run_info *io, double *step_time, uint32_t *step_nact);
 
+/* ---- Ensembles -------------------------------------------------------
+ * One context can hold K independent clusters (each its own N-body system
+ * with its own block times), integrated together: every device step
+ * advances each live cluster by one block step, in the same launches.
+ * The reference integrates one system per process; these calls have no
+ * counterpart there and serve batched ensembles of uneven sizes. */
+
+/* Per-cluster result of h4h_ensemble_run. */
+typedef struct h4h_cluster_info {
+    double   itime;         /* block time of the cluster's last step   */
+    double   atime;         /* its next block time                     */
+    uint64_t steps;         /* block steps done in this call           */
+    uint64_t interactions;  /* sum of nact * n                         */
+    uint64_t nsteps;        /* sum of nact                             */
+    int      stopped;       /* H4H_STOP_*                              */
+} h4h_cluster_info;
+
+/* Context for nclusters clusters of sizes[c] particles.  Host arrays of
+ * h4h_upload / h4h_download hold the clusters back to back in this order.
+ * Other arguments as h4h_create (same e2, eta, bounds for all clusters). */
+int h4h_create_ensemble(h4h_ctx **out, uint32_t nclusters, const uint32_t *sizes,
+                        double e2, float eta, double dt_min, double dt_max,
+                        int precision);
+
+/* Number of clusters and their sizes (sizes may be NULL). */
+int h4h_ensemble_sizes(h4h_ctx *ctx, uint32_t *nclusters, uint32_t *sizes);
+
+/* Initial forces and timesteps of every cluster at t = 0, each cluster's
+ * next block time its smallest dt (Hermite4CPU_integration.cpp:53-54). */
+int h4h_ensemble_init(h4h_ctx *ctx, float eta_s);
+
+/* Runs the reference's loop for every cluster at once (as h4h_run does for
+ * one) until each cluster has done a step with ITIME >= t_stop, reached
+ * ITIME >= t_end, or done max_steps steps.  info (nclusters entries, or
+ * NULL) receives each cluster's state. */
+int h4h_ensemble_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_steps,
+                     h4h_cluster_info *info);
+
+/* Kinetic and potential energy of every cluster (nclusters each). */
+int h4h_ensemble_energy(h4h_ctx *ctx, double *ekin, double *epot);
+
 /* Kinetic and (unsoftened) potential energy of the corrected state.
  * Replaces: NbodyUtils::get_energy (utils/NbodyUtils.cpp:427-529) and
  * Hermite4GPU::get_energy_gpu + k_energy (gpu/Hermite4GPU.cu:540-592,

@@ -375,3 +375,58 @@ def test_c1_run_mixed_energy_drift(oracle):
     print(f"C1 CumE fp64={runs['fp64'][0]:.4e} ({runs['fp64'][1]} steps) "
           f"mixed={runs['mixed'][0]:.4e} ({runs['mixed'][1]} steps)")
     assert runs["mixed"][0] <= 2.0 * runs["fp64"][0]
+
+
+# --- ensembles -----------------------------------------------------------------
+
+def test_ensemble_exact_clusters_match_oracle(oracle):
+    """Six uneven clusters in one context, exact mode, to t=0.25: every
+    cluster's schedule length, final state and energies equal the oracle's
+    run of that cluster alone."""
+    from gravidy_amd import EnsembleHIP
+    sizes = [256, 300, 512, 777, 1000, 1024]
+    systems = [ic.plummer(n, 900 + n) for n in sizes]
+    with EnsembleHIP(systems, E2, precision="exact") as ens:
+        res = ens.integration(0.25)
+        states = ens.download()
+    assert list(res["energy_time"]) == [0.0, 0.125, 0.25]
+    for c, (r, v) in enumerate(systems):
+        ref = oracle.integrate(r, v, E2, 0.25)
+        assert res["steps"][c] == len(ref["step_time"])
+        assert res["interactions"][c] == ref["interactions"]
+        assert bits_equal(states[c]["r"], ref["r"]), c
+        assert bits_equal(states[c]["v"][:, :3], ref["v"][:, :3]), c
+        assert bits_equal(states[c]["dt"], ref["dt"]) and bits_equal(states[c]["t"], ref["t"])
+        np.testing.assert_allclose(res["energy"][:, c], ref["energy"], rtol=1e-13)
+
+
+def test_ensemble_of_one_equals_single_context():
+    from gravidy_amd import EnsembleHIP
+    r, v = ic.plummer(3000, 31)
+    with Hermite4HIP(r, v, E2, precision="fp64") as h:
+        run = h.integration(0.125)
+        s1 = h.download()
+    with EnsembleHIP([(r, v)], E2, precision="fp64") as ens:
+        res = ens.integration(0.125)
+        s2 = ens.download()[0]
+    assert res["steps"][0] == len(run["step_time"])
+    assert bits_equal(s1["r"], s2["r"]) and bits_equal(s1["dt"], s2["dt"])
+    np.testing.assert_array_equal(res["energy"][:, 0], run["energy"])
+
+
+def test_ensemble_fp64_energy_errors_match_oracle(oracle):
+    """FP64 kernel, 5 clusters of 1k-4k to t=0.25: per-cluster cumulative
+    energy error agrees with the oracle's to 2 significant digits."""
+    from gravidy_amd import EnsembleHIP
+    sizes = [1024, 1500, 2048, 3000, 4096]
+    systems = [ic.plummer(n, 700 + n) for n in sizes]
+    with EnsembleHIP(systems, E2, precision="fp64") as ens:
+        res = ens.integration(0.25)
+    for c, (r, v) in enumerate(systems):
+        ref = oracle.integrate(r, v, E2, 0.25)
+        e, eo = res["energy"][:, c], ref["energy"]
+        cum = abs((e[-1] - e[0]) / e[0])
+        cum_ref = abs((eo[-1] - eo[0]) / eo[0])
+        print(f"cluster {c} n={sizes[c]}: CumE {cum:.4e} oracle {cum_ref:.4e} "
+              f"steps {res['steps'][c]}/{len(ref['step_time'])}")
+        assert abs(cum - cum_ref) <= 0.05 * cum_ref + 1e-15
