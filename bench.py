@@ -131,6 +131,44 @@ def cpu_baseline(r, v, e2, sample_nact, seed=7):
             "seconds": el}
 
 
+def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4):
+    """64 independent Plummer clusters, N_k log-uniform in [1024, 32768]
+    (seeded), each in its own Henon units, all integrated to t_end in one
+    context (one launch sequence per block step, graph-replayed)."""
+    from gravidy_amd import EnsembleHIP, ic
+    sizes = ic.log_uniform_sizes(count, seed)
+    systems = [ic.plummer(int(n), seed * 1000 + k) for k, n in enumerate(sizes)]
+    with EnsembleHIP(systems, e2, precision=precision) as ens:
+        t0 = time.perf_counter()
+        res = ens.integration(t_end)
+        el = time.perf_counter() - t0
+    e = res["energy"]
+    cum = np.abs((e[-1] - e[0]) / e[0])
+    out = {"clusters": count, "seed": seed, "n_total": int(sizes.sum()),
+           "n_min": int(sizes.min()), "n_max": int(sizes.max()), "t_end": t_end,
+           "seconds": el, "interactions": int(res["interactions"].sum()),
+           "pairs_per_s": float(res["interactions"].sum() / el),
+           "block_steps_max": int(res["steps"].max()), "block_steps_total": int(res["steps"].sum()),
+           "cum_energy_error_median": float(np.median(cum)), "cum_energy_error_max": float(cum.max())}
+    if cpu_clusters:
+        from oracle import Oracle
+        o = Oracle()
+        order = np.argsort(sizes)[:cpu_clusters]
+        t0 = time.perf_counter()
+        pairs = 0
+        for k in order:
+            r, v = systems[k]
+            ref = o.integrate(r, v, e2, 0.125)
+            pairs += ref["interactions"]
+        cel = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": pairs / cel, "unit": "pair interactions/s",
+                               "cores": o.num_threads(), "kind": "port",
+                               "sample": f"oracle integration of the {cpu_clusters} smallest "
+                                         f"clusters (n={sorted(int(sizes[k]) for k in order)}) "
+                                         f"to t=0.125, {cel:.2f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +182,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536,
                     help="active particles in the CPU baseline sample (0 = skip)")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
 
@@ -200,6 +239,12 @@ def main():
                            "mean_nact": float(np.mean(nacts)),
                            "median_nact": float(np.median(nacts))}
 
+    ens = None
+    if not args.no_ensemble:
+        log("[bench] ensemble: 64 clusters, N in [1024, 32768], to t=1")
+        ens = ensemble_leg(e2, args.precision, cpu_clusters=4 if args.cpu_sample > 0 else 0)
+        log(f"[bench] ensemble {ens['pairs_per_s']:.3e} pairs/s in {ens['seconds']:.2f} s")
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"[bench] CPU oracle baseline, nact={args.cpu_sample}")
@@ -228,6 +273,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed": mixed,
         "block_schedule": sched,
+        "ensemble": ens,
         "nact_sweep": sweep,
         "host": platform.node(),
     }

@@ -100,7 +100,9 @@ struct Ens {
     uint32_t hist_cap;
     uint32_t grid;           // persistent workgroups of the update kernel
     uint32_t max_split;
-    uint32_t pad;
+    uint32_t tuned;          // 1: tune[] holds a table for single-cluster plans
+    const uint32_t *tune;    // [64]: workgroup share [0..31], lane-group width
+                             // [32..63], by floor(log2(nact)); 0 = default
 };
 
 // Order-preserving map double -> uint64 for atomicMin.

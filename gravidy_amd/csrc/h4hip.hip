@@ -200,6 +200,11 @@ __device__ void plan_all(const Ens &e, uint32_t exact) {
             uint32_t g = (uint32_t)(share * e.grid);
             pl.grid = g < 1 ? 1 : g;
             pl.iw = pick_iw(nact ? nact : 1);
+            if (e.tuned && e.K == 1 && nact) {
+                uint32_t bucket = 31u - (uint32_t)__clz(nact);
+                if (e.tune[bucket]) pl.grid = min(e.tune[bucket], e.grid);
+                if (e.tune[32 + bucket]) pl.iw = e.tune[32 + bucket];
+            }
             Decomp d = decompose(nact, pl.iw, ntiles(e, c), pl.grid, e.max_split);
             pl.S = d.S;
             pl.tpc = d.tiles_per_chunk;
@@ -1054,6 +1059,7 @@ struct h4h_ctx {
     RunCtl *d_ctl = nullptr;
     uint32_t *d_dirty = nullptr;
     double *d_bmin = nullptr;
+    uint32_t *d_tune = nullptr;  // [64] tuning table
     double *hist_t = nullptr;
     uint32_t *hist_n = nullptr;
     // pinned host mirrors
@@ -1385,6 +1391,7 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
               hipMalloc(&ctx->d_ctl, sizeof(RunCtl)) == hipSuccess &&
               hipMalloc(&ctx->d_dirty, ctx->nblk * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&ctx->d_bmin, ctx->nblk * sizeof(double)) == hipSuccess &&
+              hipMalloc(&ctx->d_tune, 64 * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&ctx->hist_t, HIST_CAP * sizeof(double)) == hipSuccess &&
               hipMalloc(&ctx->hist_n, HIST_CAP * sizeof(uint32_t)) == hipSuccess;
     if (!ok) return cleanup(fail(H4H_ENOMEM, "device allocation failed"));
@@ -1415,6 +1422,8 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
     e.hist_t = ctx->hist_t;
     e.hist_n = ctx->hist_n;
     e.hist_cap = HIST_CAP;
+    e.tuned = 0;
+    e.tune = ctx->d_tune;
     int rc = configure_launch(ctx);
     if (rc) return cleanup(rc);
 
@@ -1443,6 +1452,7 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
          hipMemsetAsync(ctx->state, 0, np * N_SOA * sizeof(double), ctx->stream) == hipSuccess &&
          hipMemsetAsync(ctx->d_cs, 0, K * sizeof(ClState), ctx->stream) == hipSuccess &&
          hipMemsetAsync(ctx->d_ctl, 0, sizeof(RunCtl), ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->d_tune, 0, 64 * sizeof(uint32_t), ctx->stream) == hipSuccess &&
          hipMemsetAsync(ctx->d_pre, 0, 3 * (size_t)(K + 1) * sizeof(unsigned long long), ctx->stream) == hipSuccess &&
          hipStreamSynchronize(ctx->stream) == hipSuccess;
     if (!ok) return cleanup(fail(H4H_EHIP, "initialisation failed"));
@@ -1480,7 +1490,7 @@ int h4h_destroy(h4h_ctx *ctx) {
         if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
     void *dev[] = {ctx->state, ctx->staging, ctx->partial, ctx->eout, ctx->move, ctx->d_off,
                    ctx->d_cnt, ctx->d_hoff, ctx->d_bcl, ctx->d_cs, ctx->d_plan, ctx->d_pre,
-                   ctx->d_ctl, ctx->d_dirty, ctx->d_bmin, ctx->hist_t, ctx->hist_n};
+                   ctx->d_ctl, ctx->d_dirty, ctx->d_bmin, ctx->d_tune, ctx->hist_t, ctx->hist_n};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     void *host[] = {ctx->ctl_host, ctx->cs_host, ctx->hist_t_host, ctx->hist_n_host};
@@ -1927,6 +1937,87 @@ int h4h_ensemble_sizes(h4h_ctx *ctx, uint32_t *nclusters, uint32_t *sizes) {
     if (nclusters) *nclusters = ctx->K;
     if (sizes)
         for (uint32_t c = 0; c < ctx->K; ++c) sizes[c] = ctx->cnt[c];
+    return H4H_OK;
+}
+
+int h4h_set_tuning(h4h_ctx *ctx, const uint32_t *grid32, const uint32_t *iw32) {
+    int rc = check_ctx(ctx, false);
+    if (rc) return rc;
+    uint32_t table[64] = {0};
+    for (int b = 0; b < 32; ++b) {
+        uint32_t g = grid32 ? grid32[b] : 0u;
+        uint32_t w = iw32 ? iw32[b] : 0u;
+        if (w != 0 && w != 4 && w != 16 && w != 64 && w != 256)
+            return fail(H4H_EINVAL, "lane-group width must be 0, 4, 16, 64 or 256");
+        if (g > 65536) return fail(H4H_EINVAL, "workgroup share > 65536");
+        table[b] = g;
+        table[32 + b] = w;
+    }
+    H4H_HIP(hipMemcpyAsync(ctx->d_tune, table, sizeof(table), hipMemcpyHostToDevice, ctx->stream));
+    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->e.tuned = (grid32 || iw32) ? 1u : 0u;
+    drop_graph(ctx);
+    return H4H_OK;
+}
+
+// Measures the update kernel for nact = 2^b (b = 0 .. log2 n, plus n) with
+// evenly spaced active particles, for every candidate workgroup share and
+// lane-group width, and installs the fastest per bucket.  The active list
+// is left empty.
+int h4h_autotune(h4h_ctx *ctx, uint32_t reps, uint32_t *grid32, uint32_t *iw32) {
+    int rc = check_single(ctx);
+    if (rc) return rc;
+    if (reps < 1) reps = 1;
+    const uint32_t n = ctx->n;
+    const uint32_t gmax = ctx->e.grid;
+    uint32_t best_g[32] = {0}, best_w[32] = {0};
+    std::vector<uint32_t> move;
+    for (uint32_t b = 0; b < 32; ++b) {
+        uint64_t want = 1ull << b;
+        if (want > n) break;
+        uint32_t nact = (2 * want > n) ? n : (uint32_t)want;  // top bucket: all n
+        move.resize(nact);
+        for (uint32_t k = 0; k < nact; ++k) move[k] = (uint32_t)((uint64_t)k * n / nact);
+        if ((rc = h4h_set_active(ctx, move.data(), nact))) return rc;
+        std::vector<uint32_t> gs, ws;
+        for (uint32_t g = 64; g <= gmax; g *= 2) gs.push_back(g);
+        if (gs.empty() || gs.back() != gmax) gs.push_back(gmax);
+        const uint32_t wopt[4] = {4, 16, 64, 256};
+        for (uint32_t w : wopt)
+            if (w <= 4 * std::max<uint32_t>(nact, 1) || w == 256) ws.push_back(w);
+        float best = 1e30f;
+        for (uint32_t w : ws) {
+            for (uint32_t g : gs) {
+                uint32_t gt[32] = {0}, wt[32] = {0};
+                std::copy(best_g, best_g + 32, gt);
+                std::copy(best_w, best_w + 32, wt);
+                gt[b] = g;
+                wt[b] = w;
+                if ((rc = h4h_set_tuning(ctx, gt, wt))) return rc;
+                if ((rc = launch_plan(ctx))) return rc;
+                if ((rc = launch_update(ctx))) return rc;  // warm
+                H4H_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+                for (uint32_t r = 0; r < reps; ++r)
+                    if ((rc = launch_update(ctx))) return rc;
+                H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+                H4H_HIP(hipEventSynchronize(ctx->ev[1]));
+                float ms = 0.f;
+                H4H_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+                if (ms < best) {
+                    best = ms;
+                    best_g[b] = g;
+                    best_w[b] = w;
+                }
+            }
+        }
+    }
+    if ((rc = h4h_set_tuning(ctx, best_g, best_w))) return rc;
+    if (grid32) std::copy(best_g, best_g + 32, grid32);
+    if (iw32) std::copy(best_w, best_w + 32, iw32);
+    hipLaunchKernelGGL(k_set_nact0, dim3(1), dim3(1), 0, ctx->stream, ctx->e, 0u);
+    H4H_LAUNCHED();
+    H4H_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->nact_host = 0;
     return H4H_OK;
 }
 

@@ -98,6 +98,29 @@ class Hermite4HIP:
     def set_launch(self, workgroups=0, max_jsplit=0):
         check("h4h_set_launch", self.lib.h4h_set_launch(self._ctx, workgroups, max_jsplit))
 
+    def autotune(self, reps=2):
+        """Runs h4h_autotune on this system; returns {bucket: (grid, iw)}."""
+        g = np.zeros(32, np.uint32)
+        w = np.zeros(32, np.uint32)
+        u = ctypes.POINTER(ctypes.c_uint32)
+        check("h4h_autotune", self.lib.h4h_autotune(self._ctx, int(reps), g.ctypes.data_as(u),
+                                                    w.ctypes.data_as(u)))
+        return {b: (int(g[b]), int(w[b])) for b in range(32) if g[b] or w[b]}
+
+    def set_tuning(self, table):
+        """table: {bucket: (grid, iw)} as autotune returns, or None."""
+        u = ctypes.POINTER(ctypes.c_uint32)
+        if table is None:
+            check("h4h_set_tuning", self.lib.h4h_set_tuning(self._ctx, None, None))
+            return
+        g = np.zeros(32, np.uint32)
+        w = np.zeros(32, np.uint32)
+        for b, (gg, ww) in table.items():
+            g[int(b)] = gg
+            w[int(b)] = ww
+        check("h4h_set_tuning", self.lib.h4h_set_tuning(self._ctx, g.ctypes.data_as(u),
+                                                        w.ctypes.data_as(u)))
+
     def set_timing(self, enable=True):
         check("h4h_set_timing", self.lib.h4h_set_timing(self._ctx, 1 if enable else 0))
 

@@ -250,6 +250,20 @@ int h4h_ensemble_energy(h4h_ctx *ctx, double *ekin, double *epot);
  * Hermite4GPU_kernels.cu:250-288). */
 int h4h_get_energy(h4h_ctx *ctx, double *ekin, double *epot);
 
+/* Tuning table of the update kernel's work split, by bucket
+ * b = floor(log2(nact)) of a single-system context: grid32[b] is the
+ * number of workgroups the j split aims to fill (0 = the launch grid) and
+ * iw32[b] the active particles per workgroup, 4/16/64/256 (0 = default
+ * rule).  Either may be NULL; both NULL clears the table. */
+int h4h_set_tuning(h4h_ctx *ctx, const uint32_t *grid32, const uint32_t *iw32);
+
+/* Autotuner: times the update kernel on the context's own state for each
+ * bucket (nact = 2^b, evenly spaced active particles) over the candidate
+ * workgroup shares and lane-group widths, installs the fastest per bucket
+ * and returns the table (grid32, iw32: 32 entries each, may be NULL).
+ * Leaves the active list empty; call before the integration starts. */
+int h4h_autotune(h4h_ctx *ctx, uint32_t reps, uint32_t *grid32, uint32_t *iw32);
+
 /* Enables HIP-event timing of update passes and steps (stats.update_ms,
  * stats.step_ms).  Off by default. */
 int h4h_set_timing(h4h_ctx *ctx, int enable);
