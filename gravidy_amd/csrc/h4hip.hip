@@ -249,18 +249,18 @@ enum : uint32_t {
     PS_SAVE_OLD = 4,  // Hermite4::save_old_acc_jrk (Hermite4.cpp:145-153)
     PS_DEVTIME = 8,   // block time = the cluster's ATIME (device step loop)
     PS_DIRTY = 16,    // record blocks holding active particles
-    PS_PLAN = 32,     // the last workgroup computes the step's work plan
-    PS_EXACT = 64,    // plan for the FP64_EXACT kernel
 };
 
 // The active set of a cluster is appended to its segment of move[] with one
-// atomic per workgroup (ballot per wave, prefix over the waves in LDS).
-// Clusters whose loop has stopped are skipped.
+// atomic per workgroup that has active particles (ballot per wave, prefix
+// over the waves in LDS).  Clusters whose loop has stopped are skipped.
+// The work plan is made by k_plan afterwards: a last-workgroup pattern
+// here would need a device-scope fence per workgroup, i.e. an L2
+// write-back on every workgroup of this multi-XCD part.
 __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, double itime_arg,
                                               uint32_t *move) {
 #pragma clang fp contract(off)
     __shared__ uint32_t wave_off[BS / 64 + 1];
-    __shared__ uint32_t is_last;
     const uint32_t b = blockIdx.x;
     const uint32_t c = cluster_of_block(e, b);
     ClState *cs = &e.cs[c];
@@ -316,19 +316,6 @@ __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, dou
             move[e.off[c] + wave_off[BS / 64] + wave_off[w] + (uint32_t)__popcll(below)] = i;
             if (flags & PS_SAVE_OLD)
                 for (int k = 0; k < 6; ++k) s.old[k][i] = s.f[k][i];
-        }
-    }
-    if (flags & PS_PLAN) {
-        // Last workgroup to finish computes the plan from the final counts.
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0)
-            is_last = atomicAdd(&e.ctl->ticket, 1u) == gridDim.x - 1 ? 1u : 0u;
-        __syncthreads();
-        if (is_last) {
-            __threadfence();
-            plan_all(e, (flags & PS_EXACT) ? 1u : 0u);
-            if (threadIdx.x == 0) e.ctl->ticket = 0;
         }
     }
 }
@@ -1157,11 +1144,9 @@ int launch_finish(h4h_ctx *ctx, uint32_t mode) {
     return H4H_OK;
 }
 
-uint32_t exact_flag(h4h_ctx *ctx) { return ctx->precision == H4H_FP64_EXACT ? PS_EXACT : 0u; }
-
 int launch_select(h4h_ctx *ctx, uint32_t flags, double itime) {
     hipLaunchKernelGGL(k_select, dim3(ctx->nblk), dim3(BS), 0, ctx->stream, ctx->s, ctx->e,
-                       flags | exact_flag(ctx), itime, ctx->move);
+                       flags, itime, ctx->move);
     H4H_LAUNCHED();
     return H4H_OK;
 }
@@ -1183,15 +1168,14 @@ int ensure_blockmin(h4h_ctx *ctx) {
 }
 
 // One block step of every live cluster, device-driven: selection with the
-// fused predictor and save-old (its last workgroup plans the update),
-// update, finish (chunk sums + corrector), per-block minima of the touched
-// blocks, loop tail.  Optional HIP events bracket the update kernel.
+// fused predictor and save-old, work plan, update, finish (chunk sums +
+// corrector), per-block minima of the touched blocks, loop tail.  Optional HIP events bracket the update kernel.
 int enqueue_step(h4h_ctx *ctx, bool events) {
     int rc;
-    if ((rc = launch_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD | PS_DEVTIME | PS_DIRTY |
-                                     PS_PLAN,
+    if ((rc = launch_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD | PS_DEVTIME | PS_DIRTY,
                             0.0)))
         return rc;
+    if ((rc = launch_plan(ctx))) return rc;
     if (events) H4H_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     if ((rc = launch_update(ctx))) return rc;
     if (events) H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -1699,7 +1683,8 @@ int h4h_find_particles_to_move(h4h_ctx *ctx, double itime, uint32_t *nact) {
     if (!nact) return fail(H4H_EINVAL, "nact is null");
     hipLaunchKernelGGL(k_reset_select, dim3(1), dim3(1), 0, ctx->stream, ctx->e);
     H4H_LAUNCHED();
-    if ((rc = launch_select(ctx, PS_SELECT | PS_PLAN, itime))) return rc;
+    if ((rc = launch_select(ctx, PS_SELECT, itime))) return rc;
+    if ((rc = launch_plan(ctx))) return rc;
     if ((rc = read_state(ctx))) return rc;
     ctx->nact_host = ctx->cs_host[0].nact;
     *nact = ctx->nact_host;
