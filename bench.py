@@ -131,6 +131,35 @@ def cpu_baseline(r, v, e2, sample_nact, seed=7):
             "seconds": el}
 
 
+def copy_probe(mib, reps=3):
+    """Plain device-to-device copy of `mib` MiB (hipMemcpy, the runtime's
+    copy kernel), the reference point that scales rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters to bytes.  Returns GB/s of the last copy."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = int(mib) << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMemset failed")
+        hip.hipDeviceSynchronize()
+        el = 0.0
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:  # device to device
+                raise RuntimeError("hipMemcpy failed")
+            hip.hipDeviceSynchronize()
+            el = time.perf_counter() - t0
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return {"copy_mib": int(mib), "bytes_read": nbytes, "bytes_written": nbytes,
+            "gb_per_s": 2 * nbytes / el / 1e9}
+
+
 def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4):
     """64 independent Plummer clusters, N_k log-uniform in [1024, 32768]
     (seeded), each in its own Henon units, all integrated to t_end in one
@@ -184,6 +213,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -193,6 +224,10 @@ def main():
     if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
         print("bench.py: one process, one GPU only", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe > 0:
+        print(json.dumps({"copy_probe": copy_probe(args.copy_probe)}), flush=True)
+        return
 
     from gravidy_amd import Hermite4HIP, ic, softening_e2
 

@@ -521,8 +521,8 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
 // items of all clusters; each item's lane-group width comes from the plan,
 // so one launch serves every active-set size.
 template <bool MIXED>
-__global__ __launch_bounds__(BS) void k_update_fp64(Soa s, Ens e, const uint32_t *move,
-                                                   double *partial, double e2) {
+__device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const uint32_t *move,
+                                             double *partial, double e2) {
     __shared__ double4 sp[JT];
     __shared__ double4 sv[JT];
     __shared__ double red[BS * 6];
@@ -553,6 +553,20 @@ __global__ __launch_bounds__(BS) void k_update_fp64(Soa s, Ens e, const uint32_t
                 break;
         }
     }
+}
+
+// The four lane-group variants in one kernel would take 144 VGPRs (3 waves
+// per SIMD); each alone needs <= 126, so the FP64 kernel is held to 128
+// (4 waves per SIMD, no spills).  The mixed kernel spills under that bound
+// and keeps the compiler's allocation.
+__global__ __launch_bounds__(BS, 4) void k_update_fp64(Soa s, Ens e, const uint32_t *move,
+                                                      double *partial, double e2) {
+    update_items<false>(s, e, move, partial, e2);
+}
+
+__global__ __launch_bounds__(BS) void k_update_mixed(Soa s, Ens e, const uint32_t *move,
+                                                    double *partial, double e2) {
+    update_items<true>(s, e, move, partial, e2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1129,7 +1143,7 @@ int launch_update(h4h_ctx *ctx) {
         hipLaunchKernelGGL(k_update_exact, dim3(ctx->e.grid), dim3(BS), 0, ctx->stream, ctx->s,
                            ctx->e, ctx->move, ctx->e2);
     } else {
-        auto *kern = ctx->precision == H4H_MIXED ? k_update_fp64<true> : k_update_fp64<false>;
+        auto *kern = ctx->precision == H4H_MIXED ? k_update_mixed : k_update_fp64;
         hipLaunchKernelGGL(kern, dim3(ctx->e.grid), dim3(BS), 0, ctx->stream, ctx->s, ctx->e,
                            ctx->move, ctx->partial, ctx->e2);
     }
