@@ -33,6 +33,21 @@ FLOPS_PER_PAIR = 60        # convention for Hermite acc+jerk
 METRIC = "pair interactions/s & % FP64 vector peak at N=256k; mixed-vs-FP64 speedup"
 
 
+def measured_traffic(kernel, workload, precision):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a plain copy), or
+    None when no such measurement is on file."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_plummer_n262144_traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or t.get("precision") != precision:
+        return None
+    return t["bytes_per_launch"]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -212,6 +227,7 @@ def main():
                     help="active particles in the CPU baseline sample (0 = skip)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-t-end", type=float, default=1.0)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
@@ -276,8 +292,9 @@ def main():
 
     ens = None
     if not args.no_ensemble:
-        log("[bench] ensemble: 64 clusters, N in [1024, 32768], to t=1")
-        ens = ensemble_leg(e2, args.precision, cpu_clusters=4 if args.cpu_sample > 0 else 0)
+        log(f"[bench] ensemble: 64 clusters, N in [1024, 32768], to t={args.ensemble_t_end}")
+        ens = ensemble_leg(e2, args.precision, t_end=args.ensemble_t_end,
+                           cpu_clusters=4 if args.cpu_sample > 0 else 0)
         log(f"[bench] ensemble {ens['pairs_per_s']:.3e} pairs/s in {ens['seconds']:.2f} s")
 
     cpu = None
@@ -285,6 +302,8 @@ def main():
         log(f"[bench] CPU oracle baseline, nact={args.cpu_sample}")
         cpu = cpu_baseline(r, v, e2, args.cpu_sample)
 
+    kernel = {"fp64": "k_update_fp64", "exact": "k_update_exact", "mixed": "k_update_mixed"}[args.precision]
+    workload = f"plummer_n{n}_eps1e-4_eta0.01_full_block_step"
     out = {
         "metric": METRIC,
         "value": value,
@@ -298,13 +317,13 @@ def main():
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "exact": "f64", "mixed": "f32+f64"}[args.precision],
         "data": "synthetic",
-        "config": {"workload": f"plummer_n{n}_eps1e-4_eta0.01_full_block_step",
+        "config": {"workload": workload,
                    "n": n, "seed": args.seed, "precision": args.precision,
                    "shared_dt": dt_shared, "pairs_per_step": int(pairs)},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_update_fp64" if args.precision != "exact" else "k_update_exact",
-                     "flops_per_pair": FLOPS_PER_PAIR, "update_ms": upd_ms},
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic(kernel, workload, args.precision),
+                     "kernel": kernel, "flops_per_pair": FLOPS_PER_PAIR, "update_ms": upd_ms},
         "cpu_baseline": cpu,
         "mixed": mixed,
         "block_schedule": sched,

@@ -246,20 +246,12 @@ __device__ __forceinline__ float rsqrt_fp32(float x) {
     return fmaf(y * 0.5f, e, y);
 }
 
-// Mixed-precision pair: position and velocity differences in FP64 (they
-// cancel for close pairs), rounded once to FP32; r^2, 1/r, the force and
-// jerk terms in FP32, added into the FP32 partial sums t[6].  The caller
-// folds t into FP64 accumulators every few pairs.
-__device__ __forceinline__ void pair_mixed(const double pi[6], double xj,
-                                           double yj, double zj, double vxj,
-                                           double vyj, double vzj, float mj,
-                                           float e2, float t[6]) {
-    float rx = (float)(xj - pi[0]);
-    float ry = (float)(yj - pi[1]);
-    float rz = (float)(zj - pi[2]);
-    float vx = (float)(vxj - pi[3]);
-    float vy = (float)(vyj - pi[4]);
-    float vz = (float)(vzj - pi[5]);
+// FP32 pair terms from FP32 differences r = x_j - x_i, v = v_j - v_i:
+// r^2, 1/r, the force and jerk terms, added into the FP32 partial sums
+// t[6].  The caller folds t into FP64 accumulators every few pairs.
+__device__ __forceinline__ void pair_terms_f32(float rx, float ry, float rz, float vx,
+                                               float vy, float vz, float mj, float e2,
+                                               float t[6]) {
     float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, e2)));
     float rinv = rsqrt_fp32(r2);
     float rinv2 = rinv * rinv;
@@ -272,6 +264,16 @@ __device__ __forceinline__ void pair_mixed(const double pi[6], double xj,
     t[3] = fmaf(mr3, fmaf(-c, rx, vx), t[3]);
     t[4] = fmaf(mr3, fmaf(-c, ry, vy), t[4]);
     t[5] = fmaf(mr3, fmaf(-c, rz, vz), t[5]);
+}
+
+// Mixed-precision pair: position and velocity differences in FP64 (they
+// cancel for close pairs), rounded once to FP32, then pair_terms_f32.
+__device__ __forceinline__ void pair_mixed(const double pi[6], double xj,
+                                           double yj, double zj, double vxj,
+                                           double vyj, double vzj, float mj,
+                                           float e2, float t[6]) {
+    pair_terms_f32((float)(xj - pi[0]), (float)(yj - pi[1]), (float)(zj - pi[2]),
+                   (float)(vxj - pi[3]), (float)(vyj - pi[4]), (float)(vzj - pi[5]), mj, e2, t);
 }
 
 }  // namespace h4

@@ -46,6 +46,11 @@ int fail(int code, const std::string &msg) {
 #ifndef H4_MIXED_FOLD
 #define H4_MIXED_FOLD 32
 #endif
+// Unroll of the mixed inner loop: 2 keeps the kernel at 108 VGPRs (4 waves
+// per SIMD); 4 needs 132 (3 waves) and measured 1-8 % slower.
+#ifndef H4_MIXED_UNROLL
+#define H4_MIXED_UNROLL 2
+#endif
 
 constexpr uint32_t DEFAULT_GRID = 2048;       // persistent workgroups
 constexpr uint32_t DEFAULT_MAX_SPLIT = 1024;  // largest j split
@@ -385,7 +390,7 @@ __device__ __forceinline__ void tile_pairs(const double pi[6], uint32_t i,
         const float e2f = (float)e2;
         for (uint32_t q0 = 0; q0 < NQ; q0 += FOLD) {
             float t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
+#pragma unroll H4_MIXED_UNROLL
             for (uint32_t q = q0; q < q0 + FOLD; ++q) {
                 uint32_t jj = js + q * G;
                 double4 a = sp[jj];
@@ -557,8 +562,7 @@ __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const u
 
 // The four lane-group variants in one kernel would take 144 VGPRs (3 waves
 // per SIMD); each alone needs <= 126, so the FP64 kernel is held to 128
-// (4 waves per SIMD, no spills).  The mixed kernel spills under that bound
-// and keeps the compiler's allocation.
+// (4 waves per SIMD, no spills).  The mixed kernel fits in 108 by itself.
 __global__ __launch_bounds__(BS, 4) void k_update_fp64(Soa s, Ens e, const uint32_t *move,
                                                       double *partial, double e2) {
     update_items<false>(s, e, move, partial, e2);

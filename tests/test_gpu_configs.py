@@ -1,0 +1,133 @@
+"""Parity at the benchmark configurations' own recipes (through the C ABI).
+
+C2: Plummer N=65536, eps=1e-4: the FP64 path checked step by step against
+the oracle on injected states (each block step starts both sides from the
+oracle's state), from the initial conditions and again from a state the
+device loop reached on its own; energies of the same states to 1e-13.
+
+C5 recipe at reduced N: Salpeter masses on [0.1, 100] (total 1) in a
+Plummer sphere plus a central particle of mass 0.01 at rest, eps=1e-5,
+block steps down to 2^-30: the exact path bit-identical to the oracle over
+a whole run (schedule, state, energies), the FP64 path with the same
+schedule and the same cumulative energy error to 2 significant digits.
+"""
+import numpy as np
+import pytest
+
+from gravidy_amd import Hermite4HIP, ic, softening_e2
+from test_gpu_parity import bits_equal, check_forces, p6, pm_of
+
+pytestmark = pytest.mark.gpu
+
+E2 = softening_e2(1e-4)
+
+
+def oracle_step(oracle, T, r, v, f, t, dt, e2):
+    """Hermite4CPU_integration.cpp:81-124, one block step on the oracle."""
+    n = r.shape[0]
+    move = oracle.find_particles_to_move(T, t, dt)
+    p, pm = oracle.predicted_pos_vel(T, t, r, v, f)
+    fn = oracle.update_acc_jrk(move, p, pm, e2, f)
+    _, _, rn, vn, tn, dtn = oracle.correction_pos_vel(
+        T, move, p, fn, f, np.zeros((n, 3)), np.zeros((n, 3)), r, v, t, dt)
+    return move, p, pm, fn, rn, vn, tn, dtn
+
+
+def injected_steps(oracle, h, state, T, steps, e2, label):
+    """Runs `steps` block steps, each from the oracle's state uploaded to the
+    context; checks the active set, forces (1e-13, two-level), positions,
+    velocities, times, steps and the next block time.  Returns the oracle
+    state and next block time after the last step."""
+    r, v, f, t, dt = state
+    for k in range(steps):
+        h.upload(r, v, f, t, dt)
+        nact, atime = h.step(T)
+        s = h.download()
+        move, p, pm, fn, rn, vn, tn, dtn = oracle_step(oracle, T, r, v, f, t, dt, e2)
+        assert nact == move.size, (label, k, nact, move.size)
+        check_forces(s["f"], fn, move, p, pm, e2, f"{label} step {k} T={T!r} nact={nact}")
+        assert np.array_equal(s["t"], tn), (label, k)
+        assert np.array_equal(s["dt"], dtn), (label, k)
+        ar = np.abs(s["r"][:, :3] - rn[:, :3]).max() / np.abs(rn[:, :3]).max()
+        av = np.abs(s["v"][:, :3] - vn[:, :3]).max() / np.abs(vn[:, :3]).max()
+        assert ar <= 1e-15 and av <= 1e-13, (label, k, ar, av)
+        T_next = oracle.next_integration_time(tn, dtn)
+        assert atime == T_next, (label, k, atime, T_next)
+        r, v, f, t, dt, T = rn, vn, fn, tn, dtn, T_next
+    return (r, v, f, t, dt), T
+
+
+@pytest.mark.slow
+def test_c2_fp64_per_step_against_oracle(oracle):
+    n = 65536
+    r, v = ic.plummer(n, 65536)
+    fo = oracle.init_acc_jrk(p6(r, v), pm_of(r), E2)
+    to, dto, T = oracle.init_dt(fo)
+    with Hermite4HIP(r, v, E2, precision="fp64") as h:
+        h.init_acc_jrk()
+        f = h.download()["f"]
+        check_forces(f, fo, np.arange(n), p6(r, v), pm_of(r), E2, "C2 init_acc_jrk fp64")
+        ek, ep = h.get_energy()
+        eko, epo = oracle.get_energy(r, v)
+        assert ek == pytest.approx(eko, rel=1e-13) and ep == pytest.approx(epo, rel=1e-13)
+
+        state, T = injected_steps(oracle, h, (r, v, fo, to, dto), T, 24, E2, "C2 from IC")
+
+        # A state the device loop reached by itself (t = 1/16), then more
+        # injected steps from there.
+        h.upload(r, v)
+        h.init_acc_jrk()
+        atime = h.init_dt()
+        info, _, _ = h.run(0.0, atime, 0.0625, 0.0625, 1 << 20)
+        assert info["stopped"] in (1, 2)
+        s = h.download()
+        ek, ep = h.get_energy()
+        eko, epo = oracle.get_energy(s["r"], s["v"])
+        assert ek == pytest.approx(eko, rel=1e-13) and ep == pytest.approx(epo, rel=1e-13)
+        print(f"C2 device loop: {info['steps']} block steps to t={info['itime']}")
+        state = (s["r"], s["v"], s["f"], s["t"], s["dt"])
+        injected_steps(oracle, h, state, info["atime"], 16, E2, "C2 at t=1/16")
+
+
+C5_T_END = 2.0 ** -6
+C5_INTERVAL = 2.0 ** -8
+
+
+@pytest.fixture(scope="module")
+def c5_case(oracle):
+    """Reduced C5 recipe (16384 stars + the central particle) and the
+    oracle's run of it."""
+    r, v = ic.plummer_salpeter_bh(16384, 1048576)
+    e2, dt_min = softening_e2(1e-5), 2.0 ** -30
+    ref = oracle.integrate(r, v, e2, C5_T_END, interval=C5_INTERVAL, dt_min=dt_min)
+    print(f"C5 recipe n={r.shape[0]}: {len(ref['step_time'])} block steps, "
+          f"smallest dt {ref['dt'].min()!r}")
+    return r, v, e2, dt_min, ref
+
+
+@pytest.mark.slow
+def test_c5_recipe_reduced_exact_is_bitwise(c5_case):
+    r, v, e2, dt_min, ref = c5_case
+    with Hermite4HIP(r, v, e2, precision="exact", dt_min=dt_min) as h:
+        run = h.integration(C5_T_END, interval=C5_INTERVAL)
+        s = h.download()
+    assert np.array_equal(run["step_time"], ref["step_time"])
+    assert np.array_equal(run["step_nact"], ref["step_nact"])
+    assert bits_equal(s["r"], ref["r"]) and bits_equal(s["v"][:, :3], ref["v"][:, :3])
+    assert bits_equal(s["t"], ref["t"]) and bits_equal(s["dt"], ref["dt"])
+    np.testing.assert_allclose(run["energy"], ref["energy"], rtol=1e-13)
+
+
+@pytest.mark.slow
+def test_c5_recipe_reduced_fp64_energy_error_matches(c5_case):
+    r, v, e2, dt_min, ref = c5_case
+    with Hermite4HIP(r, v, e2, precision="fp64", dt_min=dt_min) as h:
+        run = h.integration(C5_T_END, interval=C5_INTERVAL)
+    same = (run["step_time"].shape == ref["step_time"].shape
+            and np.array_equal(run["step_time"], ref["step_time"])
+            and np.array_equal(run["step_nact"], ref["step_nact"]))
+    cum = abs((run["energy"][-1] - run["energy"][0]) / run["energy"][0])
+    cum_ref = abs((ref["energy"][-1] - ref["energy"][0]) / ref["energy"][0])
+    print(f"C5 recipe fp64: schedule identical={same}, CumE {cum:.6e} oracle {cum_ref:.6e}")
+    assert same
+    assert float(f"{cum:.1e}") == float(f"{cum_ref:.1e}") or abs(cum - cum_ref) <= 0.05 * cum_ref
