@@ -175,7 +175,7 @@ def copy_probe(mib, reps=3):
             "gb_per_s": 2 * nbytes / el / 1e9}
 
 
-def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4):
+def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4, graphs=True):
     """64 independent Plummer clusters, N_k log-uniform in [1024, 32768]
     (seeded), each in its own Henon units, all integrated to t_end in one
     context (one launch sequence per block step, graph-replayed)."""
@@ -183,6 +183,7 @@ def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4):
     sizes = ic.log_uniform_sizes(count, seed)
     systems = [ic.plummer(int(n), seed * 1000 + k) for k, n in enumerate(sizes)]
     with EnsembleHIP(systems, e2, precision=precision) as ens:
+        ens.set_graphs(graphs)
         t0 = time.perf_counter()
         res = ens.integration(t_end)
         el = time.perf_counter() - t0
@@ -228,6 +229,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-t-end", type=float, default=1.0)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="device loops launch kernels directly (for kernel tracers)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
@@ -284,6 +287,7 @@ def main():
         for mode, dev in (("device_loop", True), ("host_loop", False)):
             log(f"[bench] real block-timestep schedule, {args.block_steps} block steps, {mode}")
             with Hermite4HIP(r, v, e2, precision=args.precision) as h:
+                h.set_graphs(not args.no_graphs)
                 rate, el, nacts, T = block_schedule_rate(h, n, args.block_steps, dev)
             sched[mode] = {"pairs_per_s": rate, "block_steps": args.block_steps, "seconds": el,
                            "us_per_step": el / args.block_steps * 1e6, "t_reached": T,
@@ -294,7 +298,8 @@ def main():
     if not args.no_ensemble:
         log(f"[bench] ensemble: 64 clusters, N in [1024, 32768], to t={args.ensemble_t_end}")
         ens = ensemble_leg(e2, args.precision, t_end=args.ensemble_t_end,
-                           cpu_clusters=4 if args.cpu_sample > 0 else 0)
+                           cpu_clusters=4 if args.cpu_sample > 0 else 0,
+                           graphs=not args.no_graphs)
         log(f"[bench] ensemble {ens['pairs_per_s']:.3e} pairs/s in {ens['seconds']:.2f} s")
 
     cpu = None

@@ -33,7 +33,7 @@ EXPORTS = (
     "h4h_next_integration_time", "h4h_step", "h4h_run", "h4h_get_energy",
     "h4h_set_timing", "h4h_get_stats", "h4h_synchronize", "h4h_create_ensemble",
     "h4h_ensemble_sizes", "h4h_ensemble_init", "h4h_ensemble_run", "h4h_ensemble_energy",
-    "h4h_set_tuning", "h4h_autotune",
+    "h4h_set_tuning", "h4h_autotune", "h4h_set_graphs",
 )
 
 
@@ -135,6 +135,7 @@ def load():
         "h4h_ensemble_energy": ([P, pd, pd], i32),
         "h4h_set_tuning": ([P, pu, pu], i32),
         "h4h_autotune": ([P, u32, pu, pu], i32),
+        "h4h_set_graphs": ([P, i32], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -1074,6 +1074,7 @@ struct h4h_ctx {
     uint32_t *hist_n_host = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    bool graphs = true;          // device loops replay captured chunks
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     Soa s{};
     Ens e{};
@@ -1466,7 +1467,7 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
 
 extern "C" {
 
-int h4h_abi_version(void) { return 102; }
+int h4h_abi_version(void) { return 103; }
 
 const char *h4h_last_error(void) { return g_err.c_str(); }
 
@@ -1802,13 +1803,18 @@ int h4h_step(h4h_ctx *ctx, double itime, uint32_t *nact, double *atime) {
 // Replays captured chunks of RUN_CHUNK steps until every cluster's loop has
 // stopped; copies cluster 0's per-step history when asked.
 static int run_chunks(h4h_ctx *ctx, double *step_time, uint32_t *step_nact, uint64_t cap) {
-    int rc = ensure_graph(ctx);
+    int rc = ctx->graphs ? ensure_graph(ctx) : H4H_OK;
     if (rc) return rc;
     uint64_t seen = 0;
     bool hist = (step_time || step_nact) && ctx->K == 1;
     if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     for (;;) {
-        H4H_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+        if (ctx->graphs) {
+            H4H_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+        } else {
+            for (uint32_t k = 0; k < RUN_CHUNK; ++k)
+                if ((rc = enqueue_step(ctx, false))) return rc;
+        }
         H4H_HIP(hipMemcpyAsync(ctx->ctl_host, ctx->d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost,
                                ctx->stream));
         H4H_HIP(hipMemcpyAsync(ctx->cs_host, ctx->d_cs, sizeof(ClState), hipMemcpyDeviceToHost,
@@ -2021,6 +2027,14 @@ int h4h_autotune(h4h_ctx *ctx, uint32_t reps, uint32_t *grid32, uint32_t *iw32) 
     H4H_LAUNCHED();
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     ctx->nact_host = 0;
+    return H4H_OK;
+}
+
+int h4h_set_graphs(h4h_ctx *ctx, int enable) {
+    int rc = check_ctx(ctx, false);
+    if (rc) return rc;
+    ctx->graphs = enable != 0;
+    drop_graph(ctx);
     return H4H_OK;
 }
 

@@ -59,6 +59,10 @@ class EnsembleHIP:
     def set_timing(self, enable=True):
         check("h4h_set_timing", self.lib.h4h_set_timing(self._ctx, 1 if enable else 0))
 
+    def set_graphs(self, enable=True):
+        """Device loops replay captured graphs (default) or launch directly."""
+        check("h4h_set_graphs", self.lib.h4h_set_graphs(self._ctx, 1 if enable else 0))
+
     def set_launch(self, workgroups=0, max_jsplit=0):
         check("h4h_set_launch", self.lib.h4h_set_launch(self._ctx, workgroups, max_jsplit))
 

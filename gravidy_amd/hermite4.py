@@ -124,6 +124,10 @@ class Hermite4HIP:
     def set_timing(self, enable=True):
         check("h4h_set_timing", self.lib.h4h_set_timing(self._ctx, 1 if enable else 0))
 
+    def set_graphs(self, enable=True):
+        """Device loops replay captured graphs (default) or launch directly."""
+        check("h4h_set_graphs", self.lib.h4h_set_graphs(self._ctx, 1 if enable else 0))
+
     def stats(self, reset=False):
         s = _lib.H4HStats()
         check("h4h_get_stats", self.lib.h4h_get_stats(self._ctx, ctypes.byref(s), int(reset)))

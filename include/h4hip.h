@@ -268,6 +268,12 @@ int h4h_autotune(h4h_ctx *ctx, uint32_t reps, uint32_t *grid32, uint32_t *iw32);
  * stats.step_ms).  Off by default. */
 int h4h_set_timing(h4h_ctx *ctx, int enable);
 
+/* Device loops (h4h_run, h4h_ensemble_run) replay captured HIP graphs of
+ * their step chunks (default, enable = 1) or enqueue the same kernels
+ * directly (enable = 0: for tools that trace kernels one launch at a time).
+ * Results are identical either way. */
+int h4h_set_graphs(h4h_ctx *ctx, int enable);
+
 /* Reads and optionally resets the counters. */
 int h4h_get_stats(h4h_ctx *ctx, h4h_stats *out, int reset);
 

@@ -286,20 +286,24 @@ def test_c1_run_exact_is_bitwise(oracle):
 
 
 def test_device_loop_matches_host_loop(oracle):
-    """h4h_run (graph-replayed device loop) and one h4h_step call per block
-    step give the same schedule and bit-identical state, and the device
-    loop honours max_steps and the output threshold."""
+    """h4h_run (graph-replayed device loop, and the same loop launched
+    without graphs) and one h4h_step call per block step give the same
+    schedule and bit-identical state, and the device loop honours max_steps
+    and the output threshold."""
     r, v = ic.plummer(2048, 77)
     out = {}
-    for mode in ("run", "step"):
+    for mode, graphs in (("run", True), ("step", True), ("run", False)):
         with Hermite4HIP(r, v, E2, precision="fp64") as h:
+            h.set_graphs(graphs)
             res = h.integration(0.25, mode=mode)
-            out[mode] = (res, h.download())
-    (ra, sa), (rb, sb) = out["run"], out["step"]
-    assert np.array_equal(ra["step_time"], rb["step_time"])
-    assert np.array_equal(ra["step_nact"], rb["step_nact"])
-    assert bits_equal(sa["r"], sb["r"]) and bits_equal(sa["dt"], sb["dt"])
-    np.testing.assert_array_equal(ra["energy"], rb["energy"])
+            out[mode, graphs] = (res, h.download())
+    (ra, sa) = out["run", True]
+    for key in (("step", True), ("run", False)):
+        rb, sb = out[key]
+        assert np.array_equal(ra["step_time"], rb["step_time"]), key
+        assert np.array_equal(ra["step_nact"], rb["step_nact"]), key
+        assert bits_equal(sa["r"], sb["r"]) and bits_equal(sa["dt"], sb["dt"]), key
+        np.testing.assert_array_equal(ra["energy"], rb["energy"])
     with Hermite4HIP(r, v, E2, precision="fp64") as h:
         h.init_acc_jrk()
         atime = h.init_dt()
@@ -379,14 +383,16 @@ def test_c1_run_mixed_energy_drift(oracle):
 
 # --- ensembles -----------------------------------------------------------------
 
-def test_ensemble_exact_clusters_match_oracle(oracle):
+@pytest.mark.parametrize("graphs", [True, False])
+def test_ensemble_exact_clusters_match_oracle(oracle, graphs):
     """Six uneven clusters in one context, exact mode, to t=0.25: every
     cluster's schedule length, final state and energies equal the oracle's
-    run of that cluster alone."""
+    run of that cluster alone (graph-replayed and directly launched loop)."""
     from gravidy_amd import EnsembleHIP
     sizes = [256, 300, 512, 777, 1000, 1024]
     systems = [ic.plummer(n, 900 + n) for n in sizes]
     with EnsembleHIP(systems, E2, precision="exact") as ens:
+        ens.set_graphs(graphs)
         res = ens.integration(0.25)
         states = ens.download()
     assert list(res["energy_time"]) == [0.0, 0.125, 0.25]
