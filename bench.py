@@ -175,15 +175,25 @@ def copy_probe(mib, reps=3):
             "gb_per_s": 2 * nbytes / el / 1e9}
 
 
-def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4, graphs=True):
+def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4, graphs=True,
+                 tune_steps=64):
     """64 independent Plummer clusters, N_k log-uniform in [1024, 32768]
     (seeded), each in its own Henon units, all integrated to t_end in one
-    context (one launch sequence per block step, graph-replayed)."""
+    context (one launch sequence per block step, graph-replayed), with the
+    launch (persistent workgroups, j split, lane-group rule) picked by
+    h4h_tune_launch over `tune_steps` device steps from the initial state
+    (timed separately)."""
     from gravidy_amd import EnsembleHIP, ic
     sizes = ic.log_uniform_sizes(count, seed)
     systems = [ic.plummer(int(n), seed * 1000 + k) for k, n in enumerate(sizes)]
     with EnsembleHIP(systems, e2, precision=precision) as ens:
         ens.set_graphs(graphs)
+        choice, tune_s = None, 0.0
+        if tune_steps:
+            t0 = time.perf_counter()
+            ens.init()
+            choice = ens.tune_launch(tune_steps)
+            tune_s = time.perf_counter() - t0
         t0 = time.perf_counter()
         res = ens.integration(t_end)
         el = time.perf_counter() - t0
@@ -194,7 +204,8 @@ def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4, gr
            "seconds": el, "interactions": int(res["interactions"].sum()),
            "pairs_per_s": float(res["interactions"].sum() / el),
            "block_steps_max": int(res["steps"].max()), "block_steps_total": int(res["steps"].sum()),
-           "cum_energy_error_median": float(np.median(cum)), "cum_energy_error_max": float(cum.max())}
+           "cum_energy_error_median": float(np.median(cum)), "cum_energy_error_max": float(cum.max()),
+           "launch": choice, "tune_seconds": tune_s}
     if cpu_clusters:
         from oracle import Oracle
         o = Oracle()
@@ -229,6 +240,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-t-end", type=float, default=1.0)
+    ap.add_argument("--tune-steps", type=int, default=64,
+                    help="device steps per candidate of the ensemble's launch autotuner (0 = off)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="device loops launch kernels directly (for kernel tracers)")
     ap.add_argument("--no-mixed", action="store_true")
@@ -299,7 +312,7 @@ def main():
         log(f"[bench] ensemble: 64 clusters, N in [1024, 32768], to t={args.ensemble_t_end}")
         ens = ensemble_leg(e2, args.precision, t_end=args.ensemble_t_end,
                            cpu_clusters=4 if args.cpu_sample > 0 else 0,
-                           graphs=not args.no_graphs)
+                           graphs=not args.no_graphs, tune_steps=args.tune_steps)
         log(f"[bench] ensemble {ens['pairs_per_s']:.3e} pairs/s in {ens['seconds']:.2f} s")
 
     cpu = None

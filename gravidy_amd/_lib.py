@@ -33,7 +33,7 @@ EXPORTS = (
     "h4h_next_integration_time", "h4h_step", "h4h_run", "h4h_get_energy",
     "h4h_set_timing", "h4h_get_stats", "h4h_synchronize", "h4h_create_ensemble",
     "h4h_ensemble_sizes", "h4h_ensemble_init", "h4h_ensemble_run", "h4h_ensemble_energy",
-    "h4h_set_tuning", "h4h_autotune", "h4h_set_graphs",
+    "h4h_set_tuning", "h4h_autotune", "h4h_set_graphs", "h4h_tune_launch",
 )
 
 
@@ -68,6 +68,15 @@ class H4HClusterInfo(ctypes.Structure):
         ("interactions", ctypes.c_uint64),
         ("nsteps", ctypes.c_uint64),
         ("stopped", ctypes.c_int),
+    ]
+
+
+class H4HLaunchChoice(ctypes.Structure):
+    _fields_ = [
+        ("workgroups", ctypes.c_uint32),
+        ("max_jsplit", ctypes.c_uint32),
+        ("iw_fill16", ctypes.c_uint32),
+        ("ms_per_step", ctypes.c_float),
     ]
 
 
@@ -136,6 +145,7 @@ def load():
         "h4h_set_tuning": ([P, pu, pu], i32),
         "h4h_autotune": ([P, u32, pu, pu], i32),
         "h4h_set_graphs": ([P, i32], i32),
+        "h4h_tune_launch": ([P, u32, ctypes.POINTER(H4HLaunchChoice)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

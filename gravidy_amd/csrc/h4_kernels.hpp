@@ -61,7 +61,7 @@ struct ClPlan {
     uint32_t tpc;       // JT tiles per chunk
     uint32_t n_it;      // i tiles
     uint32_t rows;      // n_it * iw
-    uint32_t tree;      // finish mode: 1 = one workgroup per particle
+    uint32_t lanes;     // finish lanes per particle (finish_lanes(S))
     uint32_t grid;      // workgroup share given to decompose()
     uint32_t pad;
 };
@@ -100,6 +100,7 @@ struct Ens {
     uint32_t hist_cap;
     uint32_t grid;           // persistent workgroups of the update kernel
     uint32_t max_split;
+    uint32_t iw_fill;        // pick_iw fill rule, sixteenths
     uint32_t tuned;          // 1: tune[] holds a table for single-cluster plans
     const uint32_t *tune;    // [64]: workgroup share [0..31], lane-group width
                              // [32..63], by floor(log2(nact)); 0 = default
@@ -148,15 +149,23 @@ __host__ __device__ inline Decomp decompose(uint32_t nact, uint32_t iw,
     return d;
 }
 
+// Lanes of the finish pass that share one particle's S chunk sums: each
+// lane adds at most 8 of them (1, 4, 16 or 64 lanes).
+__host__ __device__ inline uint32_t finish_lanes(uint32_t S) {
+    return S <= 8 ? 1u : S <= 32 ? 4u : S <= 128 ? 16u : 64u;
+}
+
 // Lane-group width for nact active particles: the widest of 256/64/16/4
-// that keeps at least 3/4 of the i slots busy, else the least padded.
-__host__ __device__ inline uint32_t pick_iw(uint32_t nact) {
+// that keeps at least fill16/16 of the i slots busy (default 12/16), else
+// the least padded.
+constexpr uint32_t DEFAULT_IW_FILL = 12;
+__host__ __device__ inline uint32_t pick_iw(uint32_t nact, uint32_t fill16 = DEFAULT_IW_FILL) {
     const uint32_t opts[4] = {256, 64, 16, 4};
     uint32_t best = 256, best_pad = 0xffffffffu;
     for (int k = 0; k < 4; ++k) {
         uint32_t iw = opts[k];
         uint32_t slots = (nact + iw - 1) / iw * iw;
-        if (4ull * nact >= 3ull * slots) return iw;
+        if (16ull * nact >= (unsigned long long)fill16 * slots) return iw;
         if (slots < best_pad) {
             best_pad = slots;
             best = iw;

@@ -198,13 +198,13 @@ __device__ void plan_all(const Ens &e, uint32_t exact) {
             pl.tpc = ntiles(e, c);
             pl.n_it = (nact + BS - 1) / BS;
             pl.rows = pl.n_it * BS;
-            pl.tree = 0;
+            pl.lanes = 1;
             pl.grid = e.grid;
         } else {
             double share = W > 0.0 ? (double)nact * (double)e.cnt[c] / W : 0.0;
             uint32_t g = (uint32_t)(share * e.grid);
             pl.grid = g < 1 ? 1 : g;
-            pl.iw = pick_iw(nact ? nact : 1);
+            pl.iw = pick_iw(nact ? nact : 1, e.iw_fill);
             if (e.tuned && e.K == 1 && nact) {
                 uint32_t bucket = 31u - (uint32_t)__clz(nact);
                 if (e.tune[bucket]) pl.grid = min(e.tune[bucket], e.grid);
@@ -215,30 +215,60 @@ __device__ void plan_all(const Ens &e, uint32_t exact) {
             pl.tpc = d.tiles_per_chunk;
             pl.n_it = d.n_it;
             pl.rows = d.rows;
-            pl.tree = d.S > 32 ? 1 : 0;
+            pl.lanes = finish_lanes(d.S);
         }
         e.plan[c] = pl;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long it = 0, fu = 0, pt = 0;
-        for (uint32_t c = 0; c < e.K; ++c) {
-            e.items_pre[c] = it;
-            e.fin_pre[c] = fu;
-            e.part_pre[c] = pt;
-            uint32_t nact = e.cs[c].nact;
-            if (nact) {
-                ClPlan pl = e.plan[c];
-                it += (unsigned long long)pl.n_it * pl.S;
-                fu += pl.tree ? nact : (nact + BS - 1) / BS;
-                if (pl.S > 1) pt += 6ull * pl.S * pl.rows;
-            }
+    // Exclusive prefix sums of work items, finish units and partial doubles
+    // over the clusters, BS clusters at a time.
+    __shared__ unsigned long long wtot[3][BS / 64];
+    unsigned long long carry[3] = {0ull, 0ull, 0ull};
+    for (uint32_t base = 0; base < e.K; base += BS) {
+        const uint32_t c = base + threadIdx.x;
+        unsigned long long v[3] = {0ull, 0ull, 0ull};
+        if (c < e.K && e.cs[c].nact) {
+            const ClPlan pl = e.plan[c];
+            const uint32_t nact = e.cs[c].nact;
+            const uint32_t per = BS / pl.lanes;
+            v[0] = (unsigned long long)pl.n_it * pl.S;
+            v[1] = (nact + per - 1) / per;
+            v[2] = pl.S > 1 ? 6ull * pl.S * pl.rows : 0ull;
         }
-        e.items_pre[e.K] = it;
-        e.fin_pre[e.K] = fu;
-        e.part_pre[e.K] = pt;
-        e.ctl->items = it;
-        e.ctl->fin_units = fu;
+        unsigned long long inc[3] = {v[0], v[1], v[2]};
+        for (uint32_t o = 1; o < 64; o <<= 1)
+            for (int q = 0; q < 3; ++q) {
+                unsigned long long y = __shfl_up(inc[q], o);
+                if (__lane_id() >= o) inc[q] += y;
+            }
+        const uint32_t w = threadIdx.x >> 6;
+        if (__lane_id() == 63)
+            for (int q = 0; q < 3; ++q) wtot[q][w] = inc[q];
+        __syncthreads();
+        unsigned long long tot[3];
+        for (int q = 0; q < 3; ++q) {
+            unsigned long long before = 0ull, all = 0ull;
+            for (uint32_t k = 0; k < BS / 64; ++k) {
+                if (k < w) before += wtot[q][k];
+                all += wtot[q][k];
+            }
+            inc[q] += before + carry[q];  // inclusive, with earlier chunks
+            tot[q] = all;
+        }
+        if (c < e.K) {
+            e.items_pre[c] = inc[0] - v[0];
+            e.fin_pre[c] = inc[1] - v[1];
+            e.part_pre[c] = inc[2] - v[2];
+        }
+        for (int q = 0; q < 3; ++q) carry[q] += tot[q];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        e.items_pre[e.K] = carry[0];
+        e.fin_pre[e.K] = carry[1];
+        e.part_pre[e.K] = carry[2];
+        e.ctl->items = carry[0];
+        e.ctl->fin_units = carry[1];
     }
 }
 
@@ -668,63 +698,44 @@ __global__ __launch_bounds__(BS) void k_correct(Soa s, Ens e, const uint32_t *mo
 enum : uint32_t { FIN_WRITE_F = 0, FIN_CORRECT = 1 };
 
 // Finish pass of an update: adds the j-chunk partials of each active
-// particle in chunk order (one lane per particle when a cluster has at
-// most 32 chunks, else one workgroup per particle with a fixed-order tree)
-// and either stores the forces (FIN_WRITE_F) or applies the corrector at
-// the cluster's block time ATIME (FIN_CORRECT).
+// particle and either stores the forces (FIN_WRITE_F) or applies the
+// corrector at the cluster's block time ATIME (FIN_CORRECT).  A particle's
+// S chunk sums are spread over L = pl.lanes lanes (chunks sl, sl + L, ...
+// summed in order by lane sl, then a fixed xor butterfly), so a workgroup
+// finishes BS/L particles with no barrier; the order is fixed for a given
+// plan, so results are deterministic.
 __global__ __launch_bounds__(BS) void k_finish(Soa s, Ens e, const uint32_t *move,
                                               const double *partial, uint32_t mode, float eta,
                                               double dt_min, double dt_max) {
-    __shared__ double red[BS / 64];
     const unsigned long long total = e.ctl->fin_units;
     for (unsigned long long u = blockIdx.x; u < total; u += gridDim.x) {
         const uint32_t c = find_cluster(e.fin_pre, e.K, u);
         const ClPlan pl = e.plan[c];
+        const uint32_t L = pl.lanes;
         const uint32_t lu = (uint32_t)(u - e.fin_pre[c]);
         const uint32_t nact = e.cs[c].nact;
-        const double itime = e.cs[c].atime;
-        const uint32_t *mv = move + e.off[c];
-        const double *part = partial + e.part_pre[c];
-        if (!pl.tree) {
-            uint32_t k = lu * BS + threadIdx.x;
-            if (k < nact) {
-                uint32_t i = mv[k];
-                double f[6];
-                for (int q = 0; q < 6; ++q) {
-                    if (pl.S == 1) {
-                        f[q] = s.f[q][i];
-                    } else {
-                        double acc = 0.0;
-                        for (uint32_t sc = 0; sc < pl.S; ++sc)
-                            acc += part[((size_t)q * pl.S + sc) * pl.rows + k];
-                        f[q] = acc;
-                    }
-                }
-                if (mode == FIN_CORRECT)
-                    correct_one(s, i, f, itime, eta, dt_min, dt_max);
-                if (pl.S > 1)
-                    for (int q = 0; q < 6; ++q) s.f[q][i] = f[q];
-            }
+        const uint32_t k = lu * (BS / L) + threadIdx.x / L;
+        const uint32_t sl = threadIdx.x % L;
+        const bool valid = k < nact;
+        const uint32_t i = valid ? move[e.off[c] + k] : 0u;
+        double f[6];
+        if (pl.S == 1) {
+            for (int q = 0; q < 6; ++q) f[q] = valid ? s.f[q][i] : 0.0;
         } else {
-            uint32_t k = lu;
-            uint32_t i = mv[k];
-            double f[6];
+            const double *part = partial + e.part_pre[c];
             for (int q = 0; q < 6; ++q) {
                 double acc = 0.0;
-                for (uint32_t sc = threadIdx.x; sc < pl.S; sc += BS)
-                    acc += part[((size_t)q * pl.S + sc) * pl.rows + k];
-                for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-                __syncthreads();
-                if (__lane_id() == 0) red[threadIdx.x >> 6] = acc;
-                __syncthreads();
-                double t = red[0];
-                for (uint32_t w = 1; w < BS / 64; ++w) t += red[w];
-                f[q] = t;
+                if (valid)
+                    for (uint32_t sc = sl; sc < pl.S; sc += L)
+                        acc += part[((size_t)q * pl.S + sc) * pl.rows + k];
+                for (uint32_t o = 1; o < L; o <<= 1) acc += __shfl_xor(acc, o);
+                f[q] = acc;
             }
-            if (threadIdx.x == 0) {
-                if (mode == FIN_CORRECT) correct_one(s, i, f, itime, eta, dt_min, dt_max);
+        }
+        if (valid && sl == 0) {
+            if (mode == FIN_CORRECT) correct_one(s, i, f, e.cs[c].atime, eta, dt_min, dt_max);
+            if (pl.S > 1)
                 for (int q = 0; q < 6; ++q) s.f[q][i] = f[q];
-            }
         }
     }
 }
@@ -1044,6 +1055,7 @@ struct h4h_ctx {
     int device = 0;
     uint32_t grid = DEFAULT_GRID;      // requested persistent workgroups
     uint32_t max_split = DEFAULT_MAX_SPLIT;
+    uint32_t iw_fill = DEFAULT_IW_FILL;  // pick_iw fill rule (sixteenths)
     bool uploaded = false;
     bool timing = false;
     bool bm_valid = false;   // per-block minima match t + dt
@@ -1104,12 +1116,12 @@ uint32_t blocks_for(size_t n) { return (uint32_t)((n + BS - 1) / BS); }
 // one cluster of n particles (the persistent grid of a single-system
 // context is capped by it, so small systems do not launch idle
 // workgroups).
-uint32_t max_items(uint32_t n, uint32_t grid, uint32_t max_split) {
+uint32_t max_items(uint32_t n, uint32_t grid, uint32_t max_split, uint32_t fill16) {
     uint32_t n_jt = (n + JT - 1) / JT;
     uint32_t best = 1;
     for (uint32_t nact = 1;; nact = (nact < 64 ? nact + 1 : nact * 2)) {
         uint32_t k = nact > n ? n : nact;
-        Decomp d = decompose(k, pick_iw(k), n_jt, grid, max_split);
+        Decomp d = decompose(k, pick_iw(k, fill16), n_jt, grid, max_split);
         best = std::max(best, d.n_items);
         if (nact >= n) break;
     }
@@ -1127,9 +1139,11 @@ void drop_graph(h4h_ctx *ctx) {
 // launch fields of the cluster table.
 int configure_launch(h4h_ctx *ctx) {
     uint32_t g = ctx->grid;
-    if (ctx->K == 1) g = std::min(g, max_items(ctx->cnt[0], ctx->grid, ctx->max_split));
+    if (ctx->K == 1)
+        g = std::min(g, max_items(ctx->cnt[0], ctx->grid, ctx->max_split, ctx->iw_fill));
     ctx->e.grid = g;
     ctx->e.max_split = ctx->max_split;
+    ctx->e.iw_fill = ctx->iw_fill;
     drop_graph(ctx);
     size_t need = (size_t)6 * 2 * BS * ((size_t)g + ctx->K);
     if (ctx->partial && ctx->partial_len >= need) return H4H_OK;
@@ -1467,7 +1481,7 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
 
 extern "C" {
 
-int h4h_abi_version(void) { return 103; }
+int h4h_abi_version(void) { return 104; }
 
 const char *h4h_last_error(void) { return g_err.c_str(); }
 
@@ -2027,6 +2041,90 @@ int h4h_autotune(h4h_ctx *ctx, uint32_t reps, uint32_t *grid32, uint32_t *iw32) 
     H4H_LAUNCHED();
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     ctx->nact_host = 0;
+    return H4H_OK;
+}
+
+// Launch autotuner: every candidate (persistent workgroups x largest j
+// split x lane-group fill rule) runs the same `steps` device block steps
+// from the context's current state, which is saved once and restored
+// before each trial and at the end; the fastest is installed.
+int h4h_tune_launch(h4h_ctx *ctx, uint32_t steps, h4h_launch_choice *choice) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (steps < 1 || steps > (1u << 20)) return fail(H4H_EINVAL, "steps must be in [1, 2^20]");
+    const size_t sbytes = (size_t)ctx->n_dev * N_SOA * sizeof(double);
+    const size_t cbytes = (size_t)ctx->K * sizeof(ClState);
+    char *save = nullptr;
+    if (hipMalloc(&save, sbytes + cbytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(H4H_ENOMEM, "tune_launch: no device memory for the state copy");
+    }
+    auto copy_state = [&](bool store) -> int {
+        if (store) {
+            H4H_HIP(hipMemcpyAsync(save, ctx->state, sbytes, hipMemcpyDeviceToDevice, ctx->stream));
+            H4H_HIP(hipMemcpyAsync(save + sbytes, ctx->d_cs, cbytes, hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+        } else {
+            H4H_HIP(hipMemcpyAsync(ctx->state, save, sbytes, hipMemcpyDeviceToDevice, ctx->stream));
+            H4H_HIP(hipMemcpyAsync(ctx->d_cs, save + sbytes, cbytes, hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+            ctx->bm_valid = false;
+        }
+        return H4H_OK;
+    };
+    const h4h_stats stats0 = ctx->stats;
+    const uint32_t g0 = ctx->grid, s0 = ctx->max_split, f0 = ctx->iw_fill;
+    const uint32_t grids[5] = {256, 512, 1024, 2048, 4096};
+    const uint32_t splits[4] = {16, 64, 256, 1024};
+    const uint32_t fills[3] = {8, 12, 15};
+    float best = 1e30f;
+    uint32_t bg = g0, bs = s0, bf = f0;
+    rc = copy_state(true);
+    for (uint32_t gi = 0; gi < 5 && !rc; ++gi)
+        for (uint32_t si = 0; si < 4 && !rc; ++si)
+            for (uint32_t fi = 0; fi < 3 && !rc; ++fi) {
+                ctx->grid = grids[gi];
+                ctx->max_split = splits[si];
+                ctx->iw_fill = fills[fi];
+                if ((rc = configure_launch(ctx))) break;
+                // untimed run: graph capture and first-touch costs
+                if ((rc = copy_state(false))) break;
+                if ((rc = h4h_ensemble_run(ctx, INFINITY, INFINITY, std::min<uint32_t>(steps, RUN_CHUNK),
+                                           nullptr)))
+                    break;
+                if ((rc = copy_state(false))) break;
+                if (hipEventRecord(ctx->ev[0], ctx->stream) != hipSuccess) { rc = fail(H4H_EHIP, "event"); break; }
+                if ((rc = h4h_ensemble_run(ctx, INFINITY, INFINITY, steps, nullptr))) break;
+                float ms = 0.f;
+                if (hipEventRecord(ctx->ev[1], ctx->stream) != hipSuccess ||
+                    hipEventSynchronize(ctx->ev[1]) != hipSuccess ||
+                    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) != hipSuccess) {
+                    rc = fail(H4H_EHIP, "tune_launch: event timing failed");
+                    break;
+                }
+                if (ms < best) {
+                    best = ms;
+                    bg = grids[gi];
+                    bs = splits[si];
+                    bf = fills[fi];
+                }
+            }
+    ctx->grid = rc ? g0 : bg;
+    ctx->max_split = rc ? s0 : bs;
+    ctx->iw_fill = rc ? f0 : bf;
+    int rc2 = configure_launch(ctx);
+    if (!rc2) rc2 = copy_state(false);
+    if (!rc2 && hipStreamSynchronize(ctx->stream) != hipSuccess) rc2 = fail(H4H_EHIP, "sync");
+    (void)hipFree(save);
+    ctx->stats = stats0;
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (choice) {
+        choice->workgroups = bg;
+        choice->max_jsplit = bs;
+        choice->iw_fill16 = bf;
+        choice->ms_per_step = best / (float)steps;
+    }
     return H4H_OK;
 }
 

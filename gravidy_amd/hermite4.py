@@ -124,6 +124,14 @@ class Hermite4HIP:
     def set_timing(self, enable=True):
         check("h4h_set_timing", self.lib.h4h_set_timing(self._ctx, 1 if enable else 0))
 
+    def tune_launch(self, steps=64):
+        """Launch autotuner (h4h_tune_launch) over the next `steps` device
+        block steps from the current state (left unchanged); returns the
+        installed choice."""
+        c = _lib.H4HLaunchChoice()
+        check("h4h_tune_launch", self.lib.h4h_tune_launch(self._ctx, int(steps), ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in c._fields_}
+
     def set_graphs(self, enable=True):
         """Device loops replay captured graphs (default) or launch directly."""
         check("h4h_set_graphs", self.lib.h4h_set_graphs(self._ctx, 1 if enable else 0))

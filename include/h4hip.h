@@ -264,6 +264,24 @@ int h4h_set_tuning(h4h_ctx *ctx, const uint32_t *grid32, const uint32_t *iw32);
  * Leaves the active list empty; call before the integration starts. */
 int h4h_autotune(h4h_ctx *ctx, uint32_t reps, uint32_t *grid32, uint32_t *iw32);
 
+/* Result of h4h_tune_launch. */
+typedef struct h4h_launch_choice {
+    uint32_t workgroups;   /* persistent workgroups of the update kernel     */
+    uint32_t max_jsplit;   /* largest j split                                */
+    uint32_t iw_fill16;    /* lane-group rule: widest i-slot count keeping at
+                              least iw_fill16/16 of the slots busy           */
+    float    ms_per_step;  /* device block step time of the chosen launch    */
+} h4h_launch_choice;
+
+/* Launch autotuner for ensembles (and single systems): times `steps`
+ * device block steps (as h4h_ensemble_run) from the context's current
+ * state for every candidate persistent-workgroup count x largest j split x
+ * lane-group fill rule, restoring the state before each trial and at the
+ * end, and installs the fastest.  Call after h4h_ensemble_init (or
+ * h4h_init_dt).  Needs device memory for one copy of the state.  No
+ * counterpart in the reference, which has no tunable launch. */
+int h4h_tune_launch(h4h_ctx *ctx, uint32_t steps, h4h_launch_choice *choice);
+
 /* Enables HIP-event timing of update passes and steps (stats.update_ms,
  * stats.step_ms).  Off by default. */
 int h4h_set_timing(h4h_ctx *ctx, int enable);

@@ -131,3 +131,26 @@ def test_c5_recipe_reduced_fp64_energy_error_matches(c5_case):
     print(f"C5 recipe fp64: schedule identical={same}, CumE {cum:.6e} oracle {cum_ref:.6e}")
     assert same
     assert float(f"{cum:.1e}") == float(f"{cum_ref:.1e}") or abs(cum - cum_ref) <= 0.05 * cum_ref
+
+
+@pytest.mark.slow
+def test_c3_mixed_energy_drift_within_2x_fp64():
+    """C3: Plummer N=262144, eps=1e-4, to t=0.25 from identical initial
+    conditions: the mixed path's cumulative energy error at t_end is within
+    2x the FP64 path's (the per-particle force error of the mixed kernel at
+    this N is checked in test_gpu_parity)."""
+    import time
+    n = 262144
+    r, v = ic.plummer(n, 262144)
+    out = {}
+    for precision in ("fp64", "mixed"):
+        t0 = time.perf_counter()
+        with Hermite4HIP(r, v, E2, precision=precision) as h:
+            run = h.integration(0.25, max_steps=400000)
+        el = time.perf_counter() - t0
+        e = run["energy"]
+        assert run["energy_time"][-1] == 0.25, run["energy_time"]
+        out[precision] = abs((e[-1] - e[0]) / e[0])
+        print(f"C3 {precision}: {len(run['step_time'])} block steps in {el:.1f} s, "
+              f"{run['interactions'] / el:.3e} pairs/s, CumE {out[precision]:.3e}")
+    assert out["mixed"] <= 2.0 * out["fp64"], out

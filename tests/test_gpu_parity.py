@@ -406,6 +406,41 @@ def test_ensemble_exact_clusters_match_oracle(oracle, graphs):
         np.testing.assert_allclose(res["energy"][:, c], ref["energy"], rtol=1e-13)
 
 
+def test_tune_launch_leaves_the_state_and_results_unchanged():
+    """h4h_tune_launch runs its trial steps on a copy of the state: the
+    state after tuning is bit-identical to the state before, and an exact
+    run after tuning equals an untuned one."""
+    from gravidy_amd import EnsembleHIP
+    systems = [ic.plummer(n, 700 + n) for n in (300, 1024, 2000, 513)]
+    runs = []
+    for tune in (False, True):
+        with EnsembleHIP(systems, E2, precision="exact") as ens:
+            if tune:
+                ens.init()
+                before = ens.download()
+                choice = ens.tune_launch(32)
+                after = ens.download()
+                for a, b in zip(before, after):
+                    for key in ("r", "v", "f", "t", "dt"):
+                        assert bits_equal(a[key], b[key]), key
+                assert choice["workgroups"] in (256, 512, 1024, 2048, 4096)
+                assert choice["max_jsplit"] in (16, 64, 256, 1024)
+                assert choice["iw_fill16"] in (8, 12, 15) and choice["ms_per_step"] > 0
+            res = ens.integration(0.125)
+            runs.append((res, ens.download()))
+    (ra, sa), (rb, sb) = runs
+    assert np.array_equal(ra["steps"], rb["steps"])
+    for a, b in zip(sa, sb):
+        assert bits_equal(a["r"], b["r"]) and bits_equal(a["dt"], b["dt"])
+    with Hermite4HIP(*systems[2], E2, precision="fp64") as h:
+        h.init_acc_jrk()
+        h.init_dt()
+        s0 = h.download()
+        h.tune_launch(32)
+        s1 = h.download()
+        assert bits_equal(s0["r"], s1["r"]) and bits_equal(s0["f"], s1["f"])
+
+
 def test_ensemble_of_one_equals_single_context():
     from gravidy_amd import EnsembleHIP
     r, v = ic.plummer(3000, 31)
