@@ -67,6 +67,7 @@ class Hermite4HIP:
         self.dt_min = float(dt_min)
         self.dt_max = float(dt_max)
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self.precision = prec
         self._ctx = ctypes.c_void_p()
         check("h4h_create", self.lib.h4h_create(ctypes.byref(self._ctx), n, self.e2,
                                                 self.eta, self.dt_min, self.dt_max, prec))
@@ -94,6 +95,7 @@ class Hermite4HIP:
     def set_precision(self, precision):
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         check("h4h_set_precision", self.lib.h4h_set_precision(self._ctx, prec))
+        self.precision = prec
 
     def set_launch(self, workgroups=0, max_jsplit=0):
         check("h4h_set_launch", self.lib.h4h_set_launch(self._ctx, workgroups, max_jsplit))
@@ -263,29 +265,47 @@ class Hermite4HIP:
         check("h4h_synchronize", self.lib.h4h_synchronize(self._ctx))
 
     def integration(self, t_end, interval=0.125, eta_s=ETA_DEFAULT, max_steps=None,
-                    on_output=None, chunk_steps=65536, mode="run"):
+                    on_output=None, chunk_steps=65536, mode="run", checkpoint=None,
+                    restart=None):
         """Hermite4CPU::integration's loop (Hermite4CPU_integration.cpp:38-130)
         with every kernel on the GPU; the block steps between two output
         times run on the device (h4h_run).  integration_time and
         interval_time are float as in the reference (NbodySystem.hpp:69-70)
         and the interval test ITIME >= interval * k is evaluated in float.
         mode="step" drives the same loop one h4h_step call per block step.
-        Returns the schedule (ITIME and nact per block step) and the
-        energies at t=0 and at every output time."""
+        checkpoint: path of a binary checkpoint (gravidy_amd.checkpoint)
+        written at every output time and at the end; restart: continue from
+        such a checkpoint bit for bit (same e2, eta, bounds, precision and
+        interval) instead of starting at t = 0.
+        Returns the schedule (ITIME and nact per block step), the energies
+        at the start and at every output time, and energy_ini (the energy at
+        t = 0, carried through restarts)."""
         if mode == "step":
+            if checkpoint or restart:
+                raise ValueError("checkpoints need mode='run'")
             return self._integration_steps(t_end, interval, eta_s, max_steps, on_output)
         t_end32 = float(np.float32(t_end))
         iv32 = np.float32(interval)
-        itime = 0.0
-        self.init_acc_jrk()
-        atime = self.init_dt(1.0e10, eta_s, itime)
-        ek, ep = self.get_energy()
-        e_time, e_val = [itime], [ek + ep]
-        if on_output:
-            on_output(self, itime)
         steps_t, steps_n = [], []
-        output_factor = 1
-        interactions = 0
+        if restart:
+            c = self._read_restart(restart, float(iv32))
+            itime, atime = c["itime"], c["atime"]
+            output_factor = c["output_factor"]
+            interactions, nsteps, iterations = c["interactions"], c["nsteps"], c["iterations"]
+            energy_ini = c["energy_ini"]
+            ek, ep = self.get_energy()
+            e_time, e_val = [itime], [ek + ep]
+        else:
+            itime = 0.0
+            self.init_acc_jrk()
+            atime = self.init_dt(1.0e10, eta_s, itime)
+            ek, ep = self.get_energy()
+            energy_ini = ek + ep
+            e_time, e_val = [itime], [ek + ep]
+            if on_output:
+                on_output(self, itime)
+            output_factor = 1
+            interactions = nsteps = iterations = 0
         done = 0
         while itime < t_end32:
             budget = chunk_steps if max_steps is None else min(chunk_steps, max_steps - done)
@@ -296,7 +316,9 @@ class Hermite4HIP:
             steps_t.append(st)
             steps_n.append(sn)
             done += info["steps"]
+            iterations += info["steps"]
             interactions += info["interactions"]
+            nsteps += info["nsteps"]
             if info["steps"]:
                 itime, atime = info["itime"], info["atime"]
             if info["stopped"] == _lib.H4H_STOP_OUTPUT:
@@ -306,14 +328,43 @@ class Hermite4HIP:
                 if on_output:
                     on_output(self, itime)
                 output_factor += 1
+                if checkpoint:
+                    self._write_checkpoint(checkpoint, itime, atime, float(iv32), output_factor,
+                                           energy_ini, iterations, nsteps, interactions)
             elif info["stopped"] == _lib.H4H_STOP_END:
                 break
+        if checkpoint:
+            self._write_checkpoint(checkpoint, itime, atime, float(iv32), output_factor,
+                                   energy_ini, iterations, nsteps, interactions)
         return {
             "step_time": np.concatenate(steps_t) if steps_t else np.zeros(0),
             "step_nact": (np.concatenate(steps_n) if steps_n else np.zeros(0)).astype(np.uint32),
             "energy_time": np.array(e_time), "energy": np.array(e_val),
-            "interactions": interactions,
+            "energy_ini": energy_ini, "interactions": interactions,
         }
+
+    def _write_checkpoint(self, path, itime, atime, interval, output_factor, energy_ini,
+                          iterations, nsteps, interactions):
+        from .checkpoint import write_checkpoint
+        st = self.download()
+        st.update({"n": self.n, "e2": self.e2, "eta": self.eta, "dt_min": self.dt_min,
+                   "dt_max": self.dt_max, "precision": self.precision, "itime": itime,
+                   "atime": atime, "interval": interval, "output_factor": output_factor,
+                   "snapshot_number": output_factor, "energy_ini": energy_ini,
+                   "iterations": iterations, "nsteps": nsteps, "interactions": interactions})
+        write_checkpoint(path, st)
+
+    def _read_restart(self, path, interval):
+        from .checkpoint import CheckpointError, read_checkpoint
+        c = read_checkpoint(path)
+        for key, mine in (("n", self.n), ("e2", self.e2), ("eta", self.eta),
+                          ("dt_min", self.dt_min), ("dt_max", self.dt_max),
+                          ("precision", self.precision), ("interval", interval)):
+            if c[key] != mine:
+                raise CheckpointError(f"restart: the checkpoint's {key} ({c[key]!r}) differs "
+                                      f"from this run's ({mine!r})")
+        self.upload(c["r"], c["v"], c["f"], c["t"], c["dt"])
+        return c
 
     def _integration_steps(self, t_end, interval, eta_s, max_steps, on_output):
         t_end32 = np.float32(t_end)

@@ -18,6 +18,8 @@ int main(int argc, char *argv[]) {
     NbodyUtils nu(&ns);
 
     Hermite4HIP h4(&ns, &logger, &nu, ns.precision, ns.dt_min, ns.dt_max);
+    if (!op.checkpoint_filename.empty()) h4.set_checkpoint(op.checkpoint_filename);
+    if (!op.restart_filename.empty()) h4.set_restart(op.restart_filename);
     h4.integration();
 
     logger.log_success(std::string("Finishing..."));

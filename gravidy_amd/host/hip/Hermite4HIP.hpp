@@ -9,7 +9,10 @@
 #ifndef HERMITE4HIP_HPP
 #define HERMITE4HIP_HPP
 
+#include <string>
+
 #include "../Hermite4.hpp"
+#include "H4Checkpoint.hpp"
 #include "h4hip.h"
 
 class Hermite4HIP : public Hermite4 {
@@ -45,9 +48,25 @@ class Hermite4HIP : public Hermite4 {
 
     h4h_ctx *context() { return ctx; }
 
+    // Binary checkpoints (H4Checkpoint.hpp): written to `path` at every
+    // output time and at the end of integration(); a run started with
+    // set_restart(path) continues from one bit for bit (same e2, eta,
+    // timestep bounds, precision and interval required).
+    void set_checkpoint(const std::string &path) { checkpoint_path = path; }
+    void set_restart(const std::string &path) { restart_path = path; }
+
   private:
     h4h_ctx *ctx;
+    int precision_;
+    double dt_min_, dt_max_;
+    std::string checkpoint_path, restart_path;
     void check(int rc, const char *what);
+    void write_checkpoint(double ITIME, double ATIME, long long interactions,
+                          unsigned int nsteps, unsigned int output_factor,
+                          unsigned int snap_number);
+    void read_restart(double &ITIME, double &ATIME, long long &interactions,
+                      unsigned int &nsteps, unsigned int &output_factor,
+                      unsigned int &snap_number);
 };
 
 #endif

@@ -2,8 +2,9 @@
 // defaults (OptionsParser.cpp:41-89, 111-423): -i input, -t time (float,
 // default 1), -o output, -s softening (float, default 1e-4), -e eta (float,
 // default 0.01), -p screen, -z interval (float, default 0.125), -a all,
-// -g gpus.  Resume (-r) and Lagrange radii (-l) are not part of this build
-// and are rejected with a message.
+// -g gpus.  Resume from a text snapshot (-r) and Lagrange radii (-l) are not
+// part of this build and are rejected with a message; --checkpoint and
+// --restart give a bit-exact binary restart instead.
 #include "OptionsParser.hpp"
 
 #include <getopt.h>
@@ -36,7 +37,8 @@ OptionsParser::OptionsParser(int argc, char *argv[]) {
         {"interval", required_argument, 0, 'z'}, {"lagrange", no_argument, 0, 'l'},
         {"all", no_argument, 0, 'a'},        {"gpu", required_argument, 0, 'g'},
         {"precision", required_argument, 0, 1}, {"dt-min", required_argument, 0, 2},
-        {"dt-max", required_argument, 0, 3}, {0, 0, 0, 0}};
+        {"dt-max", required_argument, 0, 3}, {"checkpoint", required_argument, 0, 4},
+        {"restart", required_argument, 0, 5}, {0, 0, 0, 0}};
     optind = 1;
     int c;
     while ((c = getopt_long(argc, argv, "hi:t:r:o:s:e:pz:lag:", longopts, NULL)) != -1) {
@@ -56,6 +58,8 @@ OptionsParser::OptionsParser(int argc, char *argv[]) {
             case 1: opt_precision = optarg; break;
             case 2: dt_min = std::strtod(optarg, NULL); break;
             case 3: dt_max = std::strtod(optarg, NULL); break;
+            case 4: checkpoint_filename = optarg; break;
+            case 5: restart_filename = optarg; break;
             default: parse_ok = false; break;
         }
     }
@@ -83,7 +87,9 @@ void OptionsParser::usage(std::ostream &os) const {
           "  -g [ --gpu ] <value> (=0)     GPUs to use (this build: one)\n"
           "  --precision fp64|exact|mixed  Pair-kernel precision (default fp64)\n"
           "  --dt-min <value>              Smallest block timestep (default 2^-23)\n"
-          "  --dt-max <value>              Largest block timestep (default 2^-3)\n";
+          "  --dt-max <value>              Largest block timestep (default 2^-3)\n"
+          "  --checkpoint <filename>       Binary checkpoint at every output time and at the end\n"
+          "  --restart <filename>          Continue bit for bit from a binary checkpoint\n";
 }
 
 bool OptionsParser::check_options() {
@@ -125,6 +131,11 @@ bool OptionsParser::check_options() {
     }
     if (!(dt_min > 0.0) || !(dt_max >= dt_min)) {
         std::cerr << "gravidy: need 0 < dt-min <= dt-max" << std::endl;
+        return false;
+    }
+    if (!restart_filename.empty() && !file_exists(restart_filename)) {
+        std::cerr << "gravidy: cannot access " << restart_filename << ": No such file or directory"
+                  << std::endl;
         return false;
     }
     if (!(interval_time > 0.0f)) {

@@ -1,6 +1,6 @@
 // OptionsParser.hpp -- command-line options of gravidy-hip: the reference's
 // flags (OptionsParser.cpp:41-89) parsed with getopt_long, plus the
-// backend's --precision, --dt-min and --dt-max.
+// backend's --precision, --dt-min, --dt-max, --checkpoint and --restart.
 #ifndef H4_HOST_OPTIONSPARSER_HPP
 #define H4_HOST_OPTIONSPARSER_HPP
 
@@ -28,6 +28,8 @@ class OptionsParser {
     int precision;
     double dt_min;
     double dt_max;
+    std::string checkpoint_filename;  // --checkpoint: binary checkpoint at outputs
+    std::string restart_filename;     // --restart: continue from a checkpoint
 
     bool file_exists(std::string filename);
     bool check_options();

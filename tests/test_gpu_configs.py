@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from gravidy_amd import Hermite4HIP, ic, softening_e2
-from test_gpu_parity import bits_equal, check_forces, p6, pm_of
+from test_gpu_parity import bits_equal, check_forces, check_mixed, p6, pm_of
 
 pytestmark = pytest.mark.gpu
 
@@ -154,3 +154,77 @@ def test_c3_mixed_energy_drift_within_2x_fp64():
         print(f"C3 {precision}: {len(run['step_time'])} block steps in {el:.1f} s, "
               f"{run['interactions'] / el:.3e} pairs/s, CumE {out[precision]:.3e}")
     assert out["mixed"] <= 2.0 * out["fp64"], out
+
+
+@pytest.fixture(scope="module")
+def c5_full():
+    """The C5 initial conditions at full size: 1048576 Salpeter-mass stars
+    in a Plummer sphere plus the central particle (1048577 bodies)."""
+    r, v = ic.plummer_salpeter_bh(1048576, 1048576)
+    return r, v, softening_e2(1e-5)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("precision", ["exact", "fp64", "mixed"])
+def test_c5_full_size_update_on_a_sample(oracle, c5_full, precision):
+    """nact = 256 active particles (the central one and 255 seeded others)
+    against all 1048577 j at the C5 size: exact bit-identical, FP64 within
+    1e-13 (two-level), mixed within the RMS contract."""
+    r, v, e2 = c5_full
+    n = r.shape[0]
+    rng = np.random.default_rng(5)
+    move = np.sort(np.concatenate([[0], rng.choice(np.arange(1, n), 255, replace=False)]))
+    move = move.astype(np.uint32)
+    with Hermite4HIP(r, v, e2, precision=precision, dt_min=2.0 ** -30) as h:
+        h.set_active(move)
+        h.update_acc_jrk(move.size)
+        f = h.download()["f"]
+    fo = oracle.update_acc_jrk(move, p6(r, v), pm_of(r), e2, np.zeros((n, 6)))
+    label = f"C5 n={n} nact=256 {precision}"
+    if precision == "exact":
+        assert bits_equal(f[move], fo[move]), label
+    elif precision == "fp64":
+        check_forces(f, fo, move, p6(r, v), pm_of(r), e2, label)
+    else:
+        check_mixed(f, fo, move, label)
+
+
+@pytest.mark.slow
+def test_c5_full_size_initial_forces_conserve_momentum(c5_full):
+    """All 1048577 particles active (1.1e12 pairs): sum_i m_i a_i and
+    sum_i m_i j_i vanish up to rounding (pair terms are antisymmetric), a
+    size-independent check of the whole N^2 pass."""
+    r, v, e2 = c5_full
+    m = r[:, 3].astype(np.float32).astype(np.float64)
+    with Hermite4HIP(r, v, e2, precision="fp64", dt_min=2.0 ** -30) as h:
+        h.init_acc_jrk()
+        f = h.download()["f"]
+    for c, name in ((0, "acc"), (3, "jerk")):
+        net = np.abs((m[:, None] * f[:, c:c + 3]).sum(axis=0)).max()
+        scale = (m * np.linalg.norm(f[:, c:c + 3], axis=1)).sum()
+        print(f"C5 n={r.shape[0]} {name}: |sum m f| / sum m|f| = {net / scale:.3e}")
+        assert net <= 1e-12 * scale, (name, net, scale)
+
+
+@pytest.mark.parametrize("precision", ["exact", "fp64"])
+def test_python_restart_is_bit_exact(tmp_path, precision):
+    """Hermite4HIP.integration(checkpoint=..., restart=...): t = 0 -> 0.25,
+    then a new context continues from the checkpoint to 0.5; schedule and
+    final state equal the run 0 -> 0.5 in one go."""
+    r, v = ic.plummer(2048, 515)
+    with Hermite4HIP(r, v, E2, precision=precision) as h:
+        whole = h.integration(0.5)
+        sw = h.download()
+    ck = str(tmp_path / "half.ckpt")
+    with Hermite4HIP(r, v, E2, precision=precision) as h:
+        first = h.integration(0.25, checkpoint=ck)
+    with Hermite4HIP(r, v, E2, precision=precision) as h:
+        second = h.integration(0.5, restart=ck)
+        sr = h.download()
+    steps = np.concatenate([first["step_time"], second["step_time"]])
+    nact = np.concatenate([first["step_nact"], second["step_nact"]])
+    assert np.array_equal(steps, whole["step_time"]) and np.array_equal(nact, whole["step_nact"])
+    for key in ("r", "v", "f", "t", "dt"):
+        assert bits_equal(sr[key], sw[key]), key
+    assert second["energy_ini"] == whole["energy_ini"]
+    assert np.array_equal(second["energy"][1:], whole["energy"][3:])

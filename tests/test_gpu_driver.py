@@ -78,3 +78,41 @@ def test_driver_rejects_bad_input(tmp_path):
     proc = subprocess.run([DRIVER, "-i", str(tmp_path / "missing.txt")], capture_output=True,
                           text=True, timeout=60)
     assert proc.returncode != 0 and "No such file" in proc.stderr
+
+
+@pytest.mark.parametrize("precision", ["exact", "fp64"])
+def test_driver_restart_from_checkpoint_is_bit_exact(tmp_path, precision):
+    """A run to t=0.5 in one go and a run to t=0.25 continued from its
+    binary checkpoint to t=0.5 end with byte-identical checkpoints (state,
+    block times, counters) and the same energy log after the restart."""
+    r, v = ic.plummer(1024, 4242)
+    path = tmp_path / "ic.txt"
+    ic.write_ic(path, r, v)
+    base = [DRIVER, "-i", str(path), "--precision", precision]
+
+    def run(*extra, out):
+        proc = subprocess.run(base + ["-o", str(tmp_path / out)] + list(extra),
+                              capture_output=True, text=True, timeout=300)
+        assert proc.returncode == 0, proc.stderr
+        log = open(str(tmp_path / out) + ".out.log").read().splitlines()
+        return [ln.split() for ln in log if not ln.startswith("#")]  # header only at t = 0
+
+    whole = run("-t", "0.5", "--checkpoint", str(tmp_path / "whole.ckpt"), out="whole")
+    run("-t", "0.25", "--checkpoint", str(tmp_path / "half.ckpt"), out="half")
+    rest = run("-t", "0.5", "--restart", str(tmp_path / "half.ckpt"),
+               "--checkpoint", str(tmp_path / "rest.ckpt"), out="rest")
+    a = open(tmp_path / "whole.ckpt", "rb").read()
+    b = open(tmp_path / "rest.ckpt", "rb").read()
+    assert a == b
+    # time, Iter, Nsteps, Energy, RelE, CumE of the outputs after t = 0.25
+    assert [x[1:7] for x in whole[3:]] == [x[1:7] for x in rest]
+    from gravidy_amd.checkpoint import read_checkpoint
+    c = read_checkpoint(str(tmp_path / "rest.ckpt"))
+    assert c["n"] == 1024 and c["itime"] >= 0.5 and c["output_factor"] == 5
+
+    # A checkpoint of another precision is refused.
+    other = "fp64" if precision == "exact" else "exact"
+    proc = subprocess.run([DRIVER, "-i", str(path), "--precision", other, "-t", "0.5",
+                           "--restart", str(tmp_path / "half.ckpt")],
+                          capture_output=True, text=True, timeout=120)
+    assert proc.returncode != 0
