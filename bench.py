@@ -251,7 +251,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU j-split is described in DESIGN.md but not built.", file=sys.stderr)
+              "a multi-GPU j split is designed but not built.", file=sys.stderr)
         sys.exit(2)
     if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
         print("bench.py: one process, one GPU only", file=sys.stderr)
