@@ -272,6 +272,8 @@ def main():
     log(f"[bench] full block steps, precision={args.precision}")
     with Hermite4HIP(r, v, e2, precision=args.precision, dt_min=dt_shared, dt_max=dt_shared) as h:
         h.set_timing(True)
+        h.init_acc_jrk()
+        f_ref = h.download()["f"]
         sec, upd_ms, st = full_step_rate(h, n, args.steps, args.warmup)
         sweep = None
         if not args.no_sweep:
@@ -288,8 +290,17 @@ def main():
         log("[bench] mixed precision full block steps")
         with Hermite4HIP(r, v, e2, precision="mixed", dt_min=dt_shared, dt_max=dt_shared) as h:
             h.set_timing(True)
+            h.init_acc_jrk()
+            f_mix = h.download()["f"]
             msec, mupd, _ = full_step_rate(h, n, max(3, args.steps // 2), 1)
+        err = {}
+        for c, name in ((0, "acc"), (3, "jerk")):
+            rel = (np.linalg.norm(f_mix[:, c:c + 3] - f_ref[:, c:c + 3], axis=1)
+                   / np.linalg.norm(f_ref[:, c:c + 3], axis=1))
+            err[f"{name}_rel_err_rms"] = float(np.sqrt(np.mean(rel ** 2)))
+            err[f"{name}_rel_err_max"] = float(rel.max())
         mixed = {"value": pairs / msec, "ms_per_step": msec * 1e3, "update_ms": mupd,
+                 "vs_fp64_initial_forces": err,
                  "speedup_vs_fp64": (pairs / msec) / value,
                  "tflops_60": FLOPS_PER_PAIR * pairs / (mupd * 1e-3) / 1e12,
                  "frac_fp32_peak": FLOPS_PER_PAIR * pairs / (mupd * 1e-3) / 1e12 / FP32_PEAK_TFLOPS}

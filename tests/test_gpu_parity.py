@@ -406,6 +406,24 @@ def test_ensemble_exact_clusters_match_oracle(oracle, graphs):
         np.testing.assert_allclose(res["energy"][:, c], ref["energy"], rtol=1e-13)
 
 
+@pytest.mark.parametrize("sizes,t_end", [([2, 300, 3, 5, 257], 0.125), ([1, 64], 2.0 ** -16)])
+def test_ensemble_with_tiny_clusters_matches_oracle(oracle, sizes, t_end):
+    """Clusters of 1, 2, 3 and 5 particles beside larger ones.  A single
+    particle has no force, so its steps are NaN-derived and normalize_dt
+    keeps it at the smallest step (2^-23), as in the reference."""
+    from gravidy_amd import EnsembleHIP
+    systems = [ic.plummer(n, 1100 + n) for n in sizes]
+    with EnsembleHIP(systems, E2, precision="exact") as ens:
+        res = ens.integration(t_end, interval=t_end)
+        states = ens.download()
+    for c, (r, v) in enumerate(systems):
+        ref = oracle.integrate(r, v, E2, t_end, interval=t_end)
+        assert res["steps"][c] == len(ref["step_time"]), c
+        assert bits_equal(states[c]["r"], ref["r"]), c
+        assert bits_equal(states[c]["dt"], ref["dt"]) and bits_equal(states[c]["t"], ref["t"]), c
+        np.testing.assert_allclose(res["energy"][:, c], ref["energy"], rtol=1e-13, atol=1e-300)
+
+
 def test_tune_launch_leaves_the_state_and_results_unchanged():
     """h4h_tune_launch runs its trial steps on a copy of the state: the
     state after tuning is bit-identical to the state before, and an exact
