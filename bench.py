@@ -146,6 +146,61 @@ def cpu_baseline(r, v, e2, sample_nact, seed=7):
             "seconds": el}
 
 
+def run_leg(e2, n, seed, t_end, precision, cpu=False):
+    """A whole integration (device loop) of a seeded Plummer sphere: block
+    steps, interactions and the rate; with cpu=True the oracle runs the same
+    integration on the host cores and its time is reported beside it."""
+    from gravidy_amd import Hermite4HIP, ic
+    r, v = ic.plummer(n, seed)
+    with Hermite4HIP(r, v, e2, precision=precision) as h:
+        t0 = time.perf_counter()
+        run = h.integration(t_end)
+        el = time.perf_counter() - t0
+    e = run["energy"]
+    out = {"n": n, "seed": seed, "t_end": t_end, "precision": precision,
+           "block_steps": int(len(run["step_time"])), "interactions": int(run["interactions"]),
+           "seconds": el, "pairs_per_s": run["interactions"] / el,
+           "cum_energy_error": float(abs((e[-1] - e[0]) / e[0]))}
+    if cpu:
+        from oracle import Oracle
+        o = Oracle()
+        t0 = time.perf_counter()
+        ref = o.integrate(r, v, e2, t_end)
+        cel = time.perf_counter() - t0
+        out["cpu_baseline"] = {"seconds": cel, "pairs_per_s": ref["interactions"] / cel,
+                               "cores": o.num_threads(), "kind": "port",
+                               "block_steps": int(len(ref["step_time"])),
+                               "same_schedule": bool(np.array_equal(ref["step_time"],
+                                                                    run["step_time"]))}
+    return out
+
+
+def c5_run(t_end):
+    """The large mixed-precision configuration as a whole run: 1048576
+    Salpeter-mass Plummer stars plus a central particle of mass 0.01,
+    eps = 1e-5, block steps down to 2^-30, to t_end, FP64 and mixed from
+    identical initial conditions (device loop, outputs every 2^-8)."""
+    from gravidy_amd import Hermite4HIP, ic, softening_e2
+    r, v = ic.plummer_salpeter_bh(1048576, 1048576)
+    e2 = softening_e2(1e-5)
+    out = {"workload": "salpeter_plummer_bh_n1048577_eps1e-5_dtmin2^-30", "t_end": t_end}
+    for precision in ("mixed", "fp64"):
+        t0 = time.perf_counter()
+        with Hermite4HIP(r, v, e2, precision=precision, dt_min=2.0 ** -30) as h:
+            run = h.integration(t_end, interval=2.0 ** -8,
+                                on_output=lambda _h, t: log(f"[bench] C5 {precision} t={t:.6f} "
+                                                            f"{time.perf_counter() - t0:.1f} s"))
+        el = time.perf_counter() - t0
+        e = run["energy"]
+        out[precision] = {"seconds": el, "block_steps": int(len(run["step_time"])),
+                          "interactions": int(run["interactions"]),
+                          "pairs_per_s": run["interactions"] / el,
+                          "mean_nact": float(run["step_nact"].mean()),
+                          "cum_energy_error": float(abs((e[-1] - e[0]) / e[0]))}
+    out["mixed_speedup"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    return out
+
+
 def copy_probe(mib, reps=3):
     """Plain device-to-device copy of `mib` MiB (hipMemcpy, the runtime's
     copy kernel), the reference point that scales rocprofv3's FETCH_SIZE and
@@ -245,6 +300,12 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="device loops launch kernels directly (for kernel tracers)")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-runs", action="store_true",
+                    help="skip the whole-run legs (N=1024 to t=1 with the CPU oracle beside it, "
+                         "N=65536 to t=0.5)")
+    ap.add_argument("--c5-run", type=float, default=0.0, metavar="T_END",
+                    help="only run the N=1048577 Salpeter + central-mass configuration to T_END "
+                         "in mixed and FP64 (whole runs, minutes)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -259,6 +320,9 @@ def main():
 
     if args.copy_probe > 0:
         print(json.dumps({"copy_probe": copy_probe(args.copy_probe)}), flush=True)
+        return
+    if args.c5_run > 0:
+        print(json.dumps({"c5_run": c5_run(args.c5_run)}), flush=True)
         return
 
     from gravidy_amd import Hermite4HIP, ic, softening_e2
@@ -326,6 +390,13 @@ def main():
                            graphs=not args.no_graphs, tune_steps=args.tune_steps)
         log(f"[bench] ensemble {ens['pairs_per_s']:.3e} pairs/s in {ens['seconds']:.2f} s")
 
+    runs = None
+    if not args.no_runs:
+        log("[bench] whole runs: Plummer N=1024 to t=1 (+ CPU oracle), N=65536 to t=0.5")
+        runs = {"plummer_n1024_t1": run_leg(e2, 1024, 12345, 1.0, args.precision,
+                                            cpu=args.cpu_sample > 0),
+                "plummer_n65536_t0.5": run_leg(e2, 65536, 65536, 0.5, args.precision)}
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"[bench] CPU oracle baseline, nact={args.cpu_sample}")
@@ -357,6 +428,7 @@ def main():
         "mixed": mixed,
         "block_schedule": sched,
         "ensemble": ens,
+        "runs": runs,
         "nact_sweep": sweep,
         "host": platform.node(),
     }
