@@ -71,10 +71,9 @@ struct RunCtl {
     double t_end;                  // steps run while itime < t_end
     double t_stop;                 // stop after a step with itime >= t_stop
     unsigned long long max_steps;  // stop after this many steps
-    uint32_t ticket;               // last-workgroup detection in the select pass
     uint32_t ndirty;               // blocks with active particles this step
     uint32_t ndone;                // clusters that have stopped
-    uint32_t pad;
+    uint32_t pad[2];
     unsigned long long items;      // update work items this step
     unsigned long long fin_units;  // finish workgroup units this step
 };

@@ -867,7 +867,6 @@ __global__ __launch_bounds__(BS) void k_run_setup(Ens e, double t_end, double t_
         ctl.t_end = t_end;
         ctl.t_stop = t_stop;
         ctl.max_steps = max_steps;
-        ctl.ticket = 0;
         ctl.ndirty = 0;
         ctl.ndone = t;
         ctl.items = 0;
@@ -921,7 +920,6 @@ __global__ void k_reset_tmin(Ens e) {
 }
 __global__ void k_reset_select(Ens e) {
     for (uint32_t c = 0; c < e.K; ++c) e.cs[c].nact = 0;
-    e.ctl->ticket = 0;
 }
 // After init_dt: every cluster at t = itime with ATIME = its minimum dt.
 __global__ void k_set_times_from_tmin(Ens e, double itime) {
