@@ -231,7 +231,7 @@ def copy_probe(mib, reps=3):
 
 
 def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4, graphs=True,
-                 tune_steps=64):
+                 tune_steps=64, launch=None):
     """64 independent Plummer clusters, N_k log-uniform in [1024, 32768]
     (seeded), each in its own Henon units, all integrated to t_end in one
     context (one launch sequence per block step, graph-replayed), with the
@@ -244,7 +244,11 @@ def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4, gr
     with EnsembleHIP(systems, e2, precision=precision) as ens:
         ens.set_graphs(graphs)
         choice, tune_s = None, 0.0
-        if tune_steps:
+        if launch:
+            ens.set_launch_choice(*launch)
+            choice = {"workgroups": launch[0], "max_jsplit": launch[1], "iw_fill16": launch[2],
+                      "given": True}
+        elif tune_steps:
             t0 = time.perf_counter()
             ens.init()
             choice = ens.tune_launch(tune_steps)
@@ -297,6 +301,9 @@ def main():
     ap.add_argument("--ensemble-t-end", type=float, default=1.0)
     ap.add_argument("--tune-steps", type=int, default=64,
                     help="device steps per candidate of the ensemble's launch autotuner (0 = off)")
+    ap.add_argument("--ensemble-launch", default="", metavar="W,S,F",
+                    help="ensemble launch (workgroups, largest j split, lane-group fill in 16ths) "
+                         "instead of the autotuner")
     ap.add_argument("--no-graphs", action="store_true",
                     help="device loops launch kernels directly (for kernel tracers)")
     ap.add_argument("--no-mixed", action="store_true")
@@ -387,7 +394,9 @@ def main():
         log(f"[bench] ensemble: 64 clusters, N in [1024, 32768], to t={args.ensemble_t_end}")
         ens = ensemble_leg(e2, args.precision, t_end=args.ensemble_t_end,
                            cpu_clusters=4 if args.cpu_sample > 0 else 0,
-                           graphs=not args.no_graphs, tune_steps=args.tune_steps)
+                           graphs=not args.no_graphs, tune_steps=args.tune_steps,
+                           launch=[int(x) for x in args.ensemble_launch.split(",")]
+                           if args.ensemble_launch else None)
         log(f"[bench] ensemble {ens['pairs_per_s']:.3e} pairs/s in {ens['seconds']:.2f} s")
 
     runs = None

@@ -34,6 +34,7 @@ EXPORTS = (
     "h4h_set_timing", "h4h_get_stats", "h4h_synchronize", "h4h_create_ensemble",
     "h4h_ensemble_sizes", "h4h_ensemble_init", "h4h_ensemble_run", "h4h_ensemble_energy",
     "h4h_set_tuning", "h4h_autotune", "h4h_set_graphs", "h4h_tune_launch",
+    "h4h_set_launch_choice",
 )
 
 
@@ -146,6 +147,7 @@ def load():
         "h4h_autotune": ([P, u32, pu, pu], i32),
         "h4h_set_graphs": ([P, i32], i32),
         "h4h_tune_launch": ([P, u32, ctypes.POINTER(H4HLaunchChoice)], i32),
+        "h4h_set_launch_choice": ([P, ctypes.POINTER(H4HLaunchChoice)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

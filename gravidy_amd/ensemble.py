@@ -67,6 +67,11 @@ class EnsembleHIP:
         check("h4h_tune_launch", self.lib.h4h_tune_launch(self._ctx, int(steps), ctypes.byref(c)))
         return {k: getattr(c, k) for k, _ in c._fields_}
 
+    def set_launch_choice(self, workgroups, max_jsplit, iw_fill16):
+        """Installs a launch (as tune_launch returns it)."""
+        c = _lib.H4HLaunchChoice(int(workgroups), int(max_jsplit), int(iw_fill16), 0.0)
+        check("h4h_set_launch_choice", self.lib.h4h_set_launch_choice(self._ctx, ctypes.byref(c)))
+
     def set_graphs(self, enable=True):
         """Device loops replay captured graphs (default) or launch directly."""
         check("h4h_set_graphs", self.lib.h4h_set_graphs(self._ctx, 1 if enable else 0))

@@ -282,6 +282,10 @@ typedef struct h4h_launch_choice {
  * counterpart in the reference, which has no tunable launch. */
 int h4h_tune_launch(h4h_ctx *ctx, uint32_t steps, h4h_launch_choice *choice);
 
+/* Installs a launch choice (e.g. one h4h_tune_launch returned in an earlier
+ * run); ms_per_step is ignored.  No counterpart in the reference. */
+int h4h_set_launch_choice(h4h_ctx *ctx, const h4h_launch_choice *choice);
+
 /* Enables HIP-event timing of update passes and steps (stats.update_ms,
  * stats.step_ms).  Off by default. */
 int h4h_set_timing(h4h_ctx *ctx, int enable);

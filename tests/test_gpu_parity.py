@@ -446,10 +446,18 @@ def test_tune_launch_leaves_the_state_and_results_unchanged():
                 assert choice["iw_fill16"] in (8, 12, 15) and choice["ms_per_step"] > 0
             res = ens.integration(0.125)
             runs.append((res, ens.download()))
-    (ra, sa), (rb, sb) = runs
-    assert np.array_equal(ra["steps"], rb["steps"])
-    for a, b in zip(sa, sb):
-        assert bits_equal(a["r"], b["r"]) and bits_equal(a["dt"], b["dt"])
+    with EnsembleHIP(systems, E2, precision="exact") as ens:
+        ens.set_launch_choice(choice["workgroups"], choice["max_jsplit"], choice["iw_fill16"])
+        runs.append((ens.integration(0.125), ens.download()))
+        from gravidy_amd._lib import H4HError
+        for bad in ((0, 64, 12), (1024, 0, 12), (1024, 64, 0), (1024, 64, 17)):
+            with pytest.raises(H4HError):
+                ens.set_launch_choice(*bad)
+    (ra, sa) = runs[0]
+    for rb, sb in runs[1:]:
+        assert np.array_equal(ra["steps"], rb["steps"])
+        for a, b in zip(sa, sb):
+            assert bits_equal(a["r"], b["r"]) and bits_equal(a["dt"], b["dt"])
     with Hermite4HIP(*systems[2], E2, precision="fp64") as h:
         h.init_acc_jrk()
         h.init_dt()
