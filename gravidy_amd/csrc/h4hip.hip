@@ -1479,7 +1479,7 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
 
 extern "C" {
 
-int h4h_abi_version(void) { return 104; }
+int h4h_abi_version(void) { return 105; }
 
 const char *h4h_last_error(void) { return g_err.c_str(); }
 
@@ -2124,6 +2124,21 @@ int h4h_tune_launch(h4h_ctx *ctx, uint32_t steps, h4h_launch_choice *choice) {
         choice->ms_per_step = best / (float)steps;
     }
     return H4H_OK;
+}
+
+int h4h_set_launch_choice(h4h_ctx *ctx, const h4h_launch_choice *choice) {
+    int rc = check_ctx(ctx, false);
+    if (rc) return rc;
+    if (!choice) return fail(H4H_EINVAL, "choice is null");
+    if (choice->workgroups < 1 || choice->workgroups > 65536)
+        return fail(H4H_EINVAL, "workgroups must be in [1, 65536]");
+    if (choice->max_jsplit < 1) return fail(H4H_EINVAL, "max_jsplit must be >= 1");
+    if (choice->iw_fill16 < 1 || choice->iw_fill16 > 16)
+        return fail(H4H_EINVAL, "iw_fill16 must be in [1, 16]");
+    ctx->grid = choice->workgroups;
+    ctx->max_split = choice->max_jsplit;
+    ctx->iw_fill = choice->iw_fill16;
+    return configure_launch(ctx);
 }
 
 int h4h_set_graphs(h4h_ctx *ctx, int enable) {
