@@ -34,9 +34,19 @@ struct Soa {
     double *a3[3];   // 3rd derivative
     double *t;       // particle time
     double *dt;      // particle block timestep
-    double *p[6];    // predicted r[3], v[3]
-    double *pm;      // predictor mass, float-rounded (Predictor::m)
+    // Predictor (Predictor, common.hpp:213-221) as two double4 arrays, the
+    // layout of an LDS j tile: pp = (x, y, z, m) with m the float-rounded
+    // mass as a double, pv = (vx, vy, vz, bits) with the float mass's bit
+    // pattern in the low 32 bits of w (read by the mixed kernel).
+    double4 *pp;
+    double4 *pv;
 };
+
+// Float mass bit pattern stored in pv[i].w.
+__host__ __device__ inline double mass_bits(float m) {
+    unsigned long long u = __builtin_bit_cast(uint32_t, m);
+    return __builtin_bit_cast(double, u);
+}
 
 // Device-side scalars of the block-step loop of one cluster (one
 // independent N-body system; a plain context is one cluster).  Every step

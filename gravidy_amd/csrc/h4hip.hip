@@ -93,7 +93,12 @@ __device__ __forceinline__ uint32_t find_cluster(const unsigned long long *pre, 
 
 struct Cols {
     double *c[8];
+    uint32_t stride[8];  // doubles between consecutive particles (0 means 1)
 };
+
+__device__ __forceinline__ size_t col_at(const Cols &c, uint32_t k, uint32_t i) {
+    return (size_t)i * (c.stride[k] ? c.stride[k] : 1u);
+}
 
 // dst.c[k][i] = src[h*ncols + k] for the host index h of device index i.
 __global__ __launch_bounds__(BS) void k_aos_to_soa(const double *src, uint32_t ncols, Cols dst,
@@ -105,7 +110,7 @@ __global__ __launch_bounds__(BS) void k_aos_to_soa(const double *src, uint32_t n
     if (local >= e.cnt[c]) return;
     size_t h = (size_t)e.hoff[c] + local;
     for (uint32_t k = 0; k < ncols; ++k)
-        if (dst.c[k]) dst.c[k][i] = src[h * ncols + k];
+        if (dst.c[k]) dst.c[k][col_at(dst, k, i)] = src[h * ncols + k];
 }
 
 // dst[h*ncols + k] = src.c[k][i] (NULL columns write 0).
@@ -116,7 +121,8 @@ __global__ __launch_bounds__(BS) void k_soa_to_aos(Cols src, uint32_t ncols, dou
     uint32_t local = i - e.off[c];
     if (local >= e.cnt[c]) return;
     size_t h = (size_t)e.hoff[c] + local;
-    for (uint32_t k = 0; k < ncols; ++k) dst[h * ncols + k] = src.c[k] ? src.c[k][i] : 0.0;
+    for (uint32_t k = 0; k < ncols; ++k)
+        dst[h * ncols + k] = src.c[k] ? src.c[k][col_at(src, k, i)] : 0.0;
 }
 
 // Same for one array of scalars (t, dt) and for the u32 active list of a
@@ -145,17 +151,12 @@ __global__ __launch_bounds__(BS) void k_init_predictor(Soa s, Ens e) {
     if (i >= e.nblk * JT) return;
     uint32_t c = cluster_of_block(e, i / JT);
     if (i - e.off[c] < e.cnt[c]) {
-        for (int k = 0; k < 3; ++k) {
-            s.p[k][i] = s.r[k][i];
-            s.p[3 + k][i] = s.v[k][i];
-        }
-        s.pm[i] = (double)(float)s.m[i];
+        const float mf = (float)s.m[i];
+        s.pp[i] = make_double4(s.r[0][i], s.r[1][i], s.r[2][i], (double)mf);
+        s.pv[i] = make_double4(s.v[0][i], s.v[1][i], s.v[2][i], mass_bits(mf));
     } else {
-        for (int k = 0; k < 3; ++k) {
-            s.p[k][i] = PAD_POS;
-            s.p[3 + k][i] = 0.0;
-        }
-        s.pm[i] = 0.0;
+        s.pp[i] = make_double4(PAD_POS, PAD_POS, PAD_POS, 0.0);
+        s.pv[i] = make_double4(0.0, 0.0, 0.0, mass_bits(0.0f));
         s.dt[i] = INFINITY;
     }
 }
@@ -318,11 +319,14 @@ __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, dou
             double dt3 = (dt2 * dt);
             double dt2c = dt2 / 2.0;
             double dt3c = dt3 / 6.0;
+            double pr[3], pv[3];
             for (int k = 0; k < 3; ++k) {
-                s.p[k][i] = (dt3c * f[3 + k]) + (dt2c * f[k]) + (dt * v[k]) + r[k];
-                s.p[3 + k][i] = (dt2c * f[3 + k]) + (dt * f[k]) + v[k];
+                pr[k] = (dt3c * f[3 + k]) + (dt2c * f[k]) + (dt * v[k]) + r[k];
+                pv[k] = (dt2c * f[3 + k]) + (dt * f[k]) + v[k];
             }
-            s.pm[i] = (double)(float)s.m[i];
+            const float mf = (float)s.m[i];
+            s.pp[i] = make_double4(pr[0], pr[1], pr[2], (double)mf);
+            s.pv[i] = make_double4(pv[0], pv[1], pv[2], mass_bits(mf));
         }
         if (flags & PS_SELECT) {
             double tmp_time = ti + s.dt[i];
@@ -370,17 +374,48 @@ __global__ __launch_bounds__(BS) void k_save_old(Soa s, Ens e, const uint32_t *m
 // ---------------------------------------------------------------------------
 
 // Loads JT consecutive predictor records starting at j0 into LDS; each of
-// the BS (= JT) threads moves one record with coalesced SoA loads.
-// The fourth slot of sv carries the mass as a float bit pattern in its low
-// 32 bits, so the mixed-precision kernel reads it without a conversion.
+// the BS (= JT) threads moves one record (two coalesced double4 loads).
 __device__ __forceinline__ void load_jtile(const Soa &s, uint32_t j0,
                                            double4 *sp, double4 *sv) {
-    uint32_t j = j0 + threadIdx.x;
-    double m = s.pm[j];
-    unsigned long long mf = __builtin_bit_cast(uint32_t, (float)m);
-    sp[threadIdx.x] = make_double4(s.p[0][j], s.p[1][j], s.p[2][j], m);
-    sv[threadIdx.x] = make_double4(s.p[3][j], s.p[4][j], s.p[5][j],
-                                   __builtin_bit_cast(double, mf));
+    sp[threadIdx.x] = s.pp[j0 + threadIdx.x];
+    sv[threadIdx.x] = s.pv[j0 + threadIdx.x];
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) const void global_void_t;
+
+// Copies predictor records j0 .. j0 + JT - 1 into an LDS tile (pp records
+// at [0, JT), pv records at [JT, 2 JT)) with global_load_lds_dwordx4: the
+// data goes from memory to LDS without registers, so the copy of the next
+// tile runs while the workgroup computes on the current one; the barrier
+// after the computation waits for it (vmcnt).  Each wave moves 64
+// consecutive 16-byte units per instruction to consecutive LDS units.
+__device__ __forceinline__ void dma_jtile(const Soa &s, uint32_t j0, double4 *dst) {
+    const double2 *gp = reinterpret_cast<const double2 *>(s.pp + j0);
+    const double2 *gv = reinterpret_cast<const double2 *>(s.pv + j0);
+    double2 *lp = reinterpret_cast<double2 *>(dst);
+    double2 *lv = reinterpret_cast<double2 *>(dst + JT);
+    const uint32_t w = threadIdx.x & ~63u;  // first unit of this wave
+    static_assert(2 * JT == 2 * BS, "a tile is 2 x BS 16-byte units per array");
+    __builtin_amdgcn_global_load_lds((global_void_t *)(gp + threadIdx.x), (lds_void_t *)(lp + w), 16,
+                                     0, 0);
+    __builtin_amdgcn_global_load_lds((global_void_t *)(gp + BS + threadIdx.x),
+                                     (lds_void_t *)(lp + BS + w), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((global_void_t *)(gv + threadIdx.x), (lds_void_t *)(lv + w), 16,
+                                     0, 0);
+    __builtin_amdgcn_global_load_lds((global_void_t *)(gv + BS + threadIdx.x),
+                                     (lds_void_t *)(lv + BS + w), 16, 0, 0);
+}
+
+// Predicted position and velocity of particle i.
+__device__ __forceinline__ void load_predictor(const Soa &s, uint32_t i, double pi[6]) {
+    const double4 a = s.pp[i], b = s.pv[i];
+    pi[0] = a.x;
+    pi[1] = a.y;
+    pi[2] = a.z;
+    pi[3] = b.x;
+    pi[4] = b.y;
+    pi[5] = b.z;
 }
 
 __device__ __forceinline__ float mass_f32(const double4 &b) {
@@ -492,7 +527,7 @@ __global__ __launch_bounds__(BS) void k_update_exact(Soa s, Ens e, const uint32_
         bool act = k < nact;
         uint32_t i = act ? mv[k] : mv[0];
         double pi[6];
-        for (int q = 0; q < 6; ++q) pi[q] = s.p[q][i];
+        load_predictor(s, i, pi);
         double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (uint32_t j0 = j_begin; j0 < j_end; j0 += JT) {
             __syncthreads();
@@ -519,27 +554,54 @@ template <int IW, bool MIXED>
 __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint32_t jt_begin,
                                             uint32_t jt_end, uint32_t it, uint32_t sc,
                                             const uint32_t *mv, uint32_t nact, double *partial,
-                                            double e2, double4 *sp, double4 *sv, double *red) {
+                                            double e2, double4 *lA, double4 *lB, double *red) {
     const uint32_t il = threadIdx.x % IW;
     const uint32_t js = threadIdx.x / IW;
     uint32_t k = it * IW + il;
     bool act = k < nact;
     uint32_t i = act ? mv[k] : mv[0];
     double pi[6];
-    for (int q = 0; q < 6; ++q) pi[q] = s.p[q][i];
+    load_predictor(s, i, pi);
     double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     uint32_t t0 = jt_begin + sc * pl.tpc;
     uint32_t t1 = min(jt_end, t0 + pl.tpc);
-    for (uint32_t tt = t0; tt < t1; ++tt) {
-        uint32_t j0 = tt * JT;
-        __syncthreads();
-        load_jtile(s, j0, sp, sv);
-        __syncthreads();
-        bool self = (i - j0) < (uint32_t)JT;
+    auto tile = [&](const double4 *buf, uint32_t tt) {
+        const uint32_t j0 = tt * JT;
+        const bool self = (i - j0) < (uint32_t)JT;
         if (__ballot(self))
-            tile_pairs<IW, true, MIXED>(pi, i, j0, js, sp, sv, e2, fo);
+            tile_pairs<IW, true, MIXED>(pi, i, j0, js, buf, buf + JT, e2, fo);
         else
-            tile_pairs<IW, false, MIXED>(pi, i, j0, js, sp, sv, e2, fo);
+            tile_pairs<IW, false, MIXED>(pi, i, j0, js, buf, buf + JT, e2, fo);
+    };
+    // Narrow lane groups (few pairs per lane and tile): double-buffered
+    // tiles, the copy of tile tt + 1 in flight while tile tt is computed, one
+    // barrier per tile.  The loop is written out for the two buffers so that
+    // the compiler sees the copy's target and the tile being read as
+    // different LDS objects and does not wait for the copy before each
+    // read.  red overlays lA, whose last reader has passed a barrier before
+    // reduce_subgroups.
+    if constexpr (IW < 64) {
+        __syncthreads();  // the previous item is done with lA (tiles or red)
+        dma_jtile(s, t0 * JT, lA);
+        __syncthreads();
+        for (uint32_t tt = t0; tt < t1; tt += 2) {
+            if (tt + 1 < t1) dma_jtile(s, (tt + 1) * JT, lB);
+            tile(lA, tt);
+            __syncthreads();
+            if (tt + 1 >= t1) break;
+            if (tt + 2 < t1) dma_jtile(s, (tt + 2) * JT, lA);
+            tile(lB, tt + 1);
+            __syncthreads();
+        }
+    } else {
+        // Wide lane groups do 64-256 pairs per lane and tile: the copy is
+        // a small share of the time and one buffer keeps registers free.
+        for (uint32_t tt = t0; tt < t1; ++tt) {
+            __syncthreads();
+            load_jtile(s, tt * JT, lA, lA + JT);
+            __syncthreads();
+            tile(lA, tt);
+        }
     }
     reduce_subgroups<IW>(fo, red);
     if (threadIdx.x < IW && act) {
@@ -558,9 +620,10 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
 template <bool MIXED>
 __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const uint32_t *move,
                                              double *partial, double e2) {
-    __shared__ double4 sp[JT];
-    __shared__ double4 sv[JT];
-    __shared__ double red[BS * 6];
+    __shared__ double4 lA[2 * JT];
+    __shared__ double4 lB[2 * JT];
+    static_assert(BS * 6 * sizeof(double) <= sizeof(lA), "red overlays lA");
+    double *red = reinterpret_cast<double *>(lA);
     const unsigned long long total = e.ctl->items;
     for (unsigned long long item = blockIdx.x; item < total; item += gridDim.x) {
         const uint32_t c = find_cluster(e.items_pre, e.K, item);
@@ -575,16 +638,16 @@ __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const u
         double *part = partial + e.part_pre[c];
         switch (pl.iw) {
             case 4:
-                update_item<4, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, sp, sv, red);
+                update_item<4, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, lA, lB, red);
                 break;
             case 16:
-                update_item<16, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, sp, sv, red);
+                update_item<16, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, lA, lB, red);
                 break;
             case 64:
-                update_item<64, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, sp, sv, red);
+                update_item<64, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, lA, lB, red);
                 break;
             default:
-                update_item<256, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, sp, sv, red);
+                update_item<256, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, lA, lB, red);
                 break;
         }
     }
@@ -660,9 +723,11 @@ __device__ __forceinline__ void correct_one(const Soa &s, uint32_t i, const doub
         a2[c] = (-6 * (o[c] - f[c]) - dt1 * (4 * o[3 + c] + 2 * f[3 + c])) / dt2;
     for (int c = 0; c < 3; ++c)
         a3[c] = (12 * (o[c] - f[c]) + 6 * dt1 * (o[3 + c] + f[3 + c])) / dt3;
+    double pr[6];
+    load_predictor(s, i, pr);
     for (int c = 0; c < 3; ++c) {
-        s.r[c][i] = s.p[c][i] + (dt4 / 24) * a2[c] + (dt5 / 120) * a3[c];
-        s.v[c][i] = s.p[3 + c][i] + (dt3 / 6) * a2[c] + (dt4 / 24) * a3[c];
+        s.r[c][i] = pr[c] + (dt4 / 24) * a2[c] + (dt5 / 120) * a3[c];
+        s.v[c][i] = pr[3 + c] + (dt3 / 6) * a2[c] + (dt4 / 24) * a3[c];
         s.a2[c][i] = a2[c];
         s.a3[c][i] = a3[c];
     }
@@ -1458,8 +1523,11 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
     for (int c = 0; c < 3; ++c) s.a3[c] = take();
     s.t = take();
     s.dt = take();
-    for (int c = 0; c < 6; ++c) s.p[c] = take();
-    s.pm = take();
+    s.pp = reinterpret_cast<double4 *>(take());  // 4 columns each
+    for (int c = 0; c < 3; ++c) (void)take();
+    s.pv = reinterpret_cast<double4 *>(take());
+    for (int c = 0; c < 3; ++c) (void)take();
+    if (b != ctx->state + (size_t)np * N_SOA) return cleanup(fail(H4H_EINVAL, "state layout"));
     ok = hipMemcpyAsync(ctx->d_off, ctx->off.data(), K * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
          hipMemcpyAsync(ctx->d_cnt, ctx->cnt.data(), K * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
          hipMemcpyAsync(ctx->d_hoff, ctx->hoff.data(), K * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
@@ -1612,12 +1680,19 @@ int h4h_download_predictor(h4h_ctx *ctx, double *p6, float *pm) {
     const Soa &s = ctx->s;
     if (p6) {
         Cols c{};
-        for (int k = 0; k < 6; ++k) c.c[k] = s.p[k];
+        for (int k = 0; k < 3; ++k) {
+            c.c[k] = &s.pp[0].x + k;
+            c.c[3 + k] = &s.pv[0].x + k;
+            c.stride[k] = c.stride[3 + k] = 4;
+        }
         if ((rc = download_cols(ctx, p6, 6, c))) return rc;
     }
     if (pm) {
         std::vector<double> tmp(ctx->n);
-        if ((rc = download_scalar(ctx, s.pm, tmp.data()))) return rc;
+        Cols c{};
+        c.c[0] = &s.pp[0].w;
+        c.stride[0] = 4;
+        if ((rc = download_cols(ctx, tmp.data(), 1, c))) return rc;
         for (uint32_t i = 0; i < ctx->n; ++i) pm[i] = (float)tmp[i];
     }
     return H4H_OK;
@@ -1629,13 +1704,25 @@ int h4h_upload_predictor(h4h_ctx *ctx, const double *p6, const float *pm) {
     const Soa &s = ctx->s;
     if (p6) {
         Cols c{};
-        for (int k = 0; k < 6; ++k) c.c[k] = s.p[k];
+        for (int k = 0; k < 3; ++k) {
+            c.c[k] = &s.pp[0].x + k;
+            c.c[3 + k] = &s.pv[0].x + k;
+            c.stride[k] = c.stride[3 + k] = 4;
+        }
         if ((rc = upload_cols(ctx, p6, 6, c))) return rc;
     }
     if (pm) {
-        std::vector<double> tmp(ctx->n);
-        for (uint32_t i = 0; i < ctx->n; ++i) tmp[i] = (double)pm[i];
-        if ((rc = upload_scalar(ctx, tmp.data(), s.pm))) return rc;
+        // pp.w = the mass as a double, pv.w = its float bit pattern
+        std::vector<double> tmp(2 * (size_t)ctx->n);
+        for (uint32_t i = 0; i < ctx->n; ++i) {
+            tmp[2 * i] = (double)pm[i];
+            tmp[2 * i + 1] = mass_bits(pm[i]);
+        }
+        Cols c{};
+        c.c[0] = &s.pp[0].w;
+        c.c[1] = &s.pv[0].w;
+        c.stride[0] = c.stride[1] = 4;
+        if ((rc = upload_cols(ctx, tmp.data(), 2, c))) return rc;
     }
     H4H_HIP(hipStreamSynchronize(ctx->stream));
     return H4H_OK;
