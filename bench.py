@@ -175,7 +175,7 @@ def run_leg(e2, n, seed, t_end, precision, cpu=False):
     return out
 
 
-def c5_run(t_end):
+def salpeter_bh_run(t_end):
     """The large mixed-precision configuration as a whole run: 1048576
     Salpeter-mass Plummer stars plus a central particle of mass 0.01,
     eps = 1e-5, block steps down to 2^-30, to t_end, FP64 and mixed from
@@ -188,7 +188,7 @@ def c5_run(t_end):
         t0 = time.perf_counter()
         with Hermite4HIP(r, v, e2, precision=precision, dt_min=2.0 ** -30) as h:
             run = h.integration(t_end, interval=2.0 ** -8,
-                                on_output=lambda _h, t: log(f"[bench] C5 {precision} t={t:.6f} "
+                                on_output=lambda _h, t: log(f"[bench] N=1048577 {precision} t={t:.6f} "
                                                             f"{time.perf_counter() - t0:.1f} s"))
         el = time.perf_counter() - t0
         e = run["energy"]
@@ -310,7 +310,7 @@ def main():
     ap.add_argument("--no-runs", action="store_true",
                     help="skip the whole-run legs (N=1024 to t=1 with the CPU oracle beside it, "
                          "N=65536 to t=0.5)")
-    ap.add_argument("--c5-run", type=float, default=0.0, metavar="T_END",
+    ap.add_argument("--salpeter-bh-run", type=float, default=0.0, metavar="T_END",
                     help="only run the N=1048577 Salpeter + central-mass configuration to T_END "
                          "in mixed and FP64 (whole runs, minutes)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -328,8 +328,8 @@ def main():
     if args.copy_probe > 0:
         print(json.dumps({"copy_probe": copy_probe(args.copy_probe)}), flush=True)
         return
-    if args.c5_run > 0:
-        print(json.dumps({"c5_run": c5_run(args.c5_run)}), flush=True)
+    if args.salpeter_bh_run > 0:
+        print(json.dumps({"salpeter_bh_run": salpeter_bh_run(args.salpeter_bh_run)}), flush=True)
         return
 
     from gravidy_amd import Hermite4HIP, ic, softening_e2

@@ -1,11 +1,11 @@
 """Parity at the benchmark configurations' own recipes (through the C ABI).
 
-C2: Plummer N=65536, eps=1e-4: the FP64 path checked step by step against
+Plummer N=65536, eps=1e-4: the FP64 path checked step by step against
 the oracle on injected states (each block step starts both sides from the
 oracle's state), from the initial conditions and again from a state the
 device loop reached on its own; energies of the same states to 1e-13.
 
-C5 recipe at reduced N: Salpeter masses on [0.1, 100] (total 1) in a
+Salpeter masses + central particle at reduced N: Salpeter masses on [0.1, 100] (total 1) in a
 Plummer sphere plus a central particle of mass 0.01 at rest, eps=1e-5,
 block steps down to 2^-30: the exact path bit-identical to the oracle over
 a whole run (schedule, state, energies), the FP64 path with the same
@@ -58,7 +58,7 @@ def injected_steps(oracle, h, state, T, steps, e2, label):
 
 
 @pytest.mark.slow
-def test_c2_fp64_per_step_against_oracle(oracle):
+def test_plummer65536_fp64_per_step_against_oracle(oracle):
     n = 65536
     r, v = ic.plummer(n, 65536)
     fo = oracle.init_acc_jrk(p6(r, v), pm_of(r), E2)
@@ -66,12 +66,12 @@ def test_c2_fp64_per_step_against_oracle(oracle):
     with Hermite4HIP(r, v, E2, precision="fp64") as h:
         h.init_acc_jrk()
         f = h.download()["f"]
-        check_forces(f, fo, np.arange(n), p6(r, v), pm_of(r), E2, "C2 init_acc_jrk fp64")
+        check_forces(f, fo, np.arange(n), p6(r, v), pm_of(r), E2, "N=65536 init_acc_jrk fp64")
         ek, ep = h.get_energy()
         eko, epo = oracle.get_energy(r, v)
         assert ek == pytest.approx(eko, rel=1e-13) and ep == pytest.approx(epo, rel=1e-13)
 
-        state, T = injected_steps(oracle, h, (r, v, fo, to, dto), T, 24, E2, "C2 from IC")
+        state, T = injected_steps(oracle, h, (r, v, fo, to, dto), T, 24, E2, "N=65536 from IC")
 
         # A state the device loop reached by itself (t = 1/16), then more
         # injected steps from there.
@@ -84,32 +84,32 @@ def test_c2_fp64_per_step_against_oracle(oracle):
         ek, ep = h.get_energy()
         eko, epo = oracle.get_energy(s["r"], s["v"])
         assert ek == pytest.approx(eko, rel=1e-13) and ep == pytest.approx(epo, rel=1e-13)
-        print(f"C2 device loop: {info['steps']} block steps to t={info['itime']}")
+        print(f"N=65536 device loop: {info['steps']} block steps to t={info['itime']}")
         state = (s["r"], s["v"], s["f"], s["t"], s["dt"])
-        injected_steps(oracle, h, state, info["atime"], 16, E2, "C2 at t=1/16")
+        injected_steps(oracle, h, state, info["atime"], 16, E2, "N=65536 at t=1/16")
 
 
-C5_T_END = 2.0 ** -6
-C5_INTERVAL = 2.0 ** -8
+SB_T_END = 2.0 ** -6
+SB_INTERVAL = 2.0 ** -8
 
 
 @pytest.fixture(scope="module")
-def c5_case(oracle):
-    """Reduced C5 recipe (16384 stars + the central particle) and the
+def salpeter_bh_case(oracle):
+    """Salpeter + central-mass recipe at 16384 stars + the central particle, and the
     oracle's run of it."""
     r, v = ic.plummer_salpeter_bh(16384, 1048576)
     e2, dt_min = softening_e2(1e-5), 2.0 ** -30
-    ref = oracle.integrate(r, v, e2, C5_T_END, interval=C5_INTERVAL, dt_min=dt_min)
-    print(f"C5 recipe n={r.shape[0]}: {len(ref['step_time'])} block steps, "
+    ref = oracle.integrate(r, v, e2, SB_T_END, interval=SB_INTERVAL, dt_min=dt_min)
+    print(f"Salpeter+BH n={r.shape[0]}: {len(ref['step_time'])} block steps, "
           f"smallest dt {ref['dt'].min()!r}")
     return r, v, e2, dt_min, ref
 
 
 @pytest.mark.slow
-def test_c5_recipe_reduced_exact_is_bitwise(c5_case):
-    r, v, e2, dt_min, ref = c5_case
+def test_salpeter_bh_reduced_exact_is_bitwise(salpeter_bh_case):
+    r, v, e2, dt_min, ref = salpeter_bh_case
     with Hermite4HIP(r, v, e2, precision="exact", dt_min=dt_min) as h:
-        run = h.integration(C5_T_END, interval=C5_INTERVAL)
+        run = h.integration(SB_T_END, interval=SB_INTERVAL)
         s = h.download()
     assert np.array_equal(run["step_time"], ref["step_time"])
     assert np.array_equal(run["step_nact"], ref["step_nact"])
@@ -119,23 +119,23 @@ def test_c5_recipe_reduced_exact_is_bitwise(c5_case):
 
 
 @pytest.mark.slow
-def test_c5_recipe_reduced_fp64_energy_error_matches(c5_case):
-    r, v, e2, dt_min, ref = c5_case
+def test_salpeter_bh_reduced_fp64_energy_error_matches(salpeter_bh_case):
+    r, v, e2, dt_min, ref = salpeter_bh_case
     with Hermite4HIP(r, v, e2, precision="fp64", dt_min=dt_min) as h:
-        run = h.integration(C5_T_END, interval=C5_INTERVAL)
+        run = h.integration(SB_T_END, interval=SB_INTERVAL)
     same = (run["step_time"].shape == ref["step_time"].shape
             and np.array_equal(run["step_time"], ref["step_time"])
             and np.array_equal(run["step_nact"], ref["step_nact"]))
     cum = abs((run["energy"][-1] - run["energy"][0]) / run["energy"][0])
     cum_ref = abs((ref["energy"][-1] - ref["energy"][0]) / ref["energy"][0])
-    print(f"C5 recipe fp64: schedule identical={same}, CumE {cum:.6e} oracle {cum_ref:.6e}")
+    print(f"Salpeter+BH fp64: schedule identical={same}, CumE {cum:.6e} oracle {cum_ref:.6e}")
     assert same
     assert float(f"{cum:.1e}") == float(f"{cum_ref:.1e}") or abs(cum - cum_ref) <= 0.05 * cum_ref
 
 
 @pytest.mark.slow
-def test_c3_mixed_energy_drift_within_2x_fp64():
-    """C3: Plummer N=262144, eps=1e-4, to t=0.25 from identical initial
+def test_plummer262144_mixed_energy_drift_within_2x_fp64():
+    """Plummer N=262144, eps=1e-4, to t=0.25 from identical initial
     conditions: the mixed path's cumulative energy error at t_end is within
     2x the FP64 path's (the per-particle force error of the mixed kernel at
     this N is checked in test_gpu_parity)."""
@@ -151,14 +151,14 @@ def test_c3_mixed_energy_drift_within_2x_fp64():
         e = run["energy"]
         assert run["energy_time"][-1] == 0.25, run["energy_time"]
         out[precision] = abs((e[-1] - e[0]) / e[0])
-        print(f"C3 {precision}: {len(run['step_time'])} block steps in {el:.1f} s, "
+        print(f"N=262144 {precision}: {len(run['step_time'])} block steps in {el:.1f} s, "
               f"{run['interactions'] / el:.3e} pairs/s, CumE {out[precision]:.3e}")
     assert out["mixed"] <= 2.0 * out["fp64"], out
 
 
 @pytest.fixture(scope="module")
-def c5_full():
-    """The C5 initial conditions at full size: 1048576 Salpeter-mass stars
+def salpeter_bh_full():
+    """The Salpeter + central-mass initial conditions at full size: 1048576 Salpeter-mass stars
     in a Plummer sphere plus the central particle (1048577 bodies)."""
     r, v = ic.plummer_salpeter_bh(1048576, 1048576)
     return r, v, softening_e2(1e-5)
@@ -166,11 +166,11 @@ def c5_full():
 
 @pytest.mark.slow
 @pytest.mark.parametrize("precision", ["exact", "fp64", "mixed"])
-def test_c5_full_size_update_on_a_sample(oracle, c5_full, precision):
+def test_salpeter_bh_full_size_update_on_a_sample(oracle, salpeter_bh_full, precision):
     """nact = 256 active particles (the central one and 255 seeded others)
-    against all 1048577 j at the C5 size: exact bit-identical, FP64 within
+    against all 1048577 j: exact bit-identical, FP64 within
     1e-13 (two-level), mixed within the RMS contract."""
-    r, v, e2 = c5_full
+    r, v, e2 = salpeter_bh_full
     n = r.shape[0]
     rng = np.random.default_rng(5)
     move = np.sort(np.concatenate([[0], rng.choice(np.arange(1, n), 255, replace=False)]))
@@ -180,7 +180,7 @@ def test_c5_full_size_update_on_a_sample(oracle, c5_full, precision):
         h.update_acc_jrk(move.size)
         f = h.download()["f"]
     fo = oracle.update_acc_jrk(move, p6(r, v), pm_of(r), e2, np.zeros((n, 6)))
-    label = f"C5 n={n} nact=256 {precision}"
+    label = f"Salpeter+BH n={n} nact=256 {precision}"
     if precision == "exact":
         assert bits_equal(f[move], fo[move]), label
     elif precision == "fp64":
@@ -190,11 +190,11 @@ def test_c5_full_size_update_on_a_sample(oracle, c5_full, precision):
 
 
 @pytest.mark.slow
-def test_c5_full_size_initial_forces_conserve_momentum(c5_full):
+def test_salpeter_bh_full_size_initial_forces_conserve_momentum(salpeter_bh_full):
     """All 1048577 particles active (1.1e12 pairs): sum_i m_i a_i and
     sum_i m_i j_i vanish up to rounding (pair terms are antisymmetric), a
     size-independent check of the whole N^2 pass."""
-    r, v, e2 = c5_full
+    r, v, e2 = salpeter_bh_full
     m = r[:, 3].astype(np.float32).astype(np.float64)
     with Hermite4HIP(r, v, e2, precision="fp64", dt_min=2.0 ** -30) as h:
         h.init_acc_jrk()
@@ -202,7 +202,7 @@ def test_c5_full_size_initial_forces_conserve_momentum(c5_full):
     for c, name in ((0, "acc"), (3, "jerk")):
         net = np.abs((m[:, None] * f[:, c:c + 3]).sum(axis=0)).max()
         scale = (m * np.linalg.norm(f[:, c:c + 3], axis=1)).sum()
-        print(f"C5 n={r.shape[0]} {name}: |sum m f| / sum m|f| = {net / scale:.3e}")
+        print(f"Salpeter+BH n={r.shape[0]} {name}: |sum m f| / sum m|f| = {net / scale:.3e}")
         assert net <= 1e-12 * scale, (name, net, scale)
 
 

@@ -23,7 +23,7 @@ LOG_HEADER = ("#".ljust(2) + "IteTime".rjust(10) + "Iter".rjust(15) + "Nsteps".r
 
 def _run(tmp_path, precision):
     r, v = ic.plummer(1024, 12345)
-    path = tmp_path / "c1.txt"
+    path = tmp_path / "p1024.txt"
     ic.write_ic(path, r, v)
     proc = subprocess.run([DRIVER, "-i", str(path), "-t", "1", "--precision", precision],
                           capture_output=True, text=True, timeout=300)
@@ -32,7 +32,7 @@ def _run(tmp_path, precision):
 
 
 @pytest.mark.parametrize("precision", ["exact", "fp64"])
-def test_driver_c1_outputs(oracle, tmp_path, precision):
+def test_driver_plummer1024_outputs(oracle, tmp_path, precision):
     r, v, out = _run(tmp_path, precision)
     ref = oracle.integrate(r, v, softening_e2(1e-4), 1.0)
 
@@ -60,7 +60,7 @@ def test_driver_c1_outputs(oracle, tmp_path, precision):
 
     # snapshots _00 (t=0) ... _08 (t=1) and the final one _09
     names = sorted(f for f in os.listdir(os.path.dirname(out)) if ".snapshot_" in f)
-    assert names == [f"c1.txt.out.snapshot_{k:02d}" for k in range(10)]
+    assert names == [f"p1024.txt.out.snapshot_{k:02d}" for k in range(10)]
     _, r9, v9, _ = ic.read_ic(out + ".snapshot_09")
     # 7 significant digits (scientific, precision 6)
     np.testing.assert_allclose(r9[:, :3], ref["r"][:, :3], rtol=1e-6, atol=1e-6)

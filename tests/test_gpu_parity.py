@@ -16,7 +16,7 @@ bound of a floating-point sum.
 MIXED (FP32 pair terms): every pair term carries a few FP32 roundings
 (~1e-7 of itself), so the per-particle error is a statistic over
 particles: the root-mean-square of |a_gpu - a_ref| / |a_ref| must be
-<= 1e-7 (the north star's "relative force error") and that of the jerk
+<= 1e-7 (the target relative force error) and that of the jerk
 <= 5e-7; median, 99th percentile and maximum are printed.  Its cumulative
 energy error over a whole run must stay within 2x the FP64 path's.  Energies are compared to 1e-13 relative (only the
 summation order differs).
@@ -268,7 +268,7 @@ def test_energy_matches(oracle, n):
 
 # --- whole runs -------------------------------------------------------------
 
-def test_c1_run_exact_is_bitwise(oracle):
+def test_plummer1024_run_exact_is_bitwise(oracle):
     """Plummer N=1024, eps=1e-4, eta=0.01, dt in [2^-23, 2^-3], to t=1:
     identical block-timestep schedule and bit-identical final state."""
     r, v = ic.plummer(1024, 12345)
@@ -315,7 +315,7 @@ def test_device_loop_matches_host_loop(oracle):
         assert np.array_equal(st, ra["step_time"][5:5 + len(st)])
 
 
-def test_c1_run_fp64_energy_error_matches(oracle):
+def test_plummer1024_run_fp64_energy_error_matches(oracle):
     """Fast FP64 kernel over the same run: the cumulative energy error at
     t_end (CumE, Logger.cpp:328) agrees with the oracle's to 2 significant
     digits; the schedule agreement is reported."""
@@ -328,7 +328,7 @@ def test_c1_run_fp64_energy_error_matches(oracle):
             and np.array_equal(run["step_nact"], ref["step_nact"]))
     cum = abs((run["energy"][-1] - run["energy"][0]) / run["energy"][0])
     cum_ref = abs((ref["energy"][-1] - ref["energy"][0]) / ref["energy"][0])
-    print(f"fp64 C1: schedule identical={same} steps={len(run['step_time'])}/"
+    print(f"fp64 N=1024: schedule identical={same} steps={len(run['step_time'])}/"
           f"{len(ref['step_time'])} CumE={cum:.6e} oracle CumE={cum_ref:.6e}")
     assert float(f"{cum:.1e}") == float(f"{cum_ref:.1e}") or abs(cum - cum_ref) <= 0.05 * cum_ref
 
@@ -366,7 +366,7 @@ def test_update_mixed_within_tolerance(oracle, n, nact):
     check_mixed(f, fo, move, f"update n={n} nact={nact} mixed")
 
 
-def test_c1_run_mixed_energy_drift(oracle):
+def test_plummer1024_run_mixed_energy_drift(oracle):
     """Plummer N=1024 to t=1 with the mixed kernel: the cumulative energy
     error at t_end stays within 2x of the FP64 path's."""
     r, v = ic.plummer(1024, 12345)
@@ -376,7 +376,7 @@ def test_c1_run_mixed_energy_drift(oracle):
             run = h.integration(1.0)
         e = run["energy"]
         runs[prec] = (abs((e[-1] - e[0]) / e[0]), len(run["step_time"]))
-    print(f"C1 CumE fp64={runs['fp64'][0]:.4e} ({runs['fp64'][1]} steps) "
+    print(f"N=1024 CumE fp64={runs['fp64'][0]:.4e} ({runs['fp64'][1]} steps) "
           f"mixed={runs['mixed'][0]:.4e} ({runs['mixed'][1]} steps)")
     assert runs["mixed"][0] <= 2.0 * runs["fp64"][0]
 
