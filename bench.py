@@ -294,7 +294,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "exact", "mixed"])
     ap.add_argument("--block-steps", type=int, default=200,
                     help="real-schedule block steps to time (0 = skip)")
-    ap.add_argument("--cpu-sample", type=int, default=65536,
+    ap.add_argument("--cpu-sample", type=int, default=131072,
                     help="active particles in the CPU baseline sample (0 = skip)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")

@@ -208,6 +208,32 @@ def test_update_fp64_jsplit_within_tolerance(oracle, n, nact):
     assert np.all(f[rest] == 0.0)
 
 
+def test_autotune_table_is_valid_and_keeps_results_within_tolerance(oracle):
+    """h4h_autotune picks a (workgroup share, lane-group width) per log2(nact)
+    bucket on the context's own particles; forces computed under the table
+    stay within the FP64 bound, set_tuning(None) restores the default rule
+    and invalid widths are refused."""
+    from gravidy_amd._lib import H4HError
+    n = 8192
+    r, v = ic.plummer(n, 818)
+    rng = np.random.default_rng(5)
+    with Hermite4HIP(r, v, E2, precision="fp64") as h:
+        table = h.autotune(reps=1)
+        assert set(table) == set(range(14)), table  # nact = 1, 2, ..., 8192
+        for b, (g, w) in table.items():
+            assert w in (4, 16, 64, 256) and 64 <= g <= 65536, (b, g, w)
+        for nact in (3, 100, 2000, n):
+            move = np.sort(rng.choice(n, size=nact, replace=False)).astype(np.uint32)
+            h.set_active(move)
+            h.update_acc_jrk(nact)
+            f = h.download()["f"]
+            fo = oracle.update_acc_jrk(move, p6(r, v), pm_of(r), E2, np.zeros((n, 6)))
+            check_forces(f, fo, move, p6(r, v), pm_of(r), E2, f"tuned nact={nact}")
+        h.set_tuning(None)
+        with pytest.raises(H4HError):
+            h.set_tuning({3: (512, 8)})
+
+
 # --- initial timesteps ----------------------------------------------------
 
 def _exact_power_inputs():
