@@ -179,6 +179,62 @@ __global__ __launch_bounds__(BS) void k_identity_active(uint32_t *move, Ens e) {
 // count; the update workgroups are shared in proportion to nact * n.
 // Exclusive prefix sums over clusters go to items_pre / fin_pre /
 // part_pre, totals to ctl.  `exact`: one lane per i, no j split.
+// Work plan of cluster c for nact active particles; W = sum over the
+// clusters of nact * n (the update workgroups are shared in proportion).
+__device__ __forceinline__ ClPlan plan_cluster(const Ens &e, uint32_t c, uint32_t nact, double W,
+                                               uint32_t exact) {
+    ClPlan pl;
+    pl.pad = 0;
+    if (exact) {
+        pl.iw = BS;
+        pl.S = 1;
+        pl.tpc = ntiles(e, c);
+        pl.n_it = (nact + BS - 1) / BS;
+        pl.rows = pl.n_it * BS;
+        pl.lanes = 1;
+        pl.grid = e.grid;
+    } else {
+        double share = W > 0.0 ? (double)nact * (double)e.cnt[c] / W : 0.0;
+        uint32_t g = (uint32_t)(share * e.grid);
+        pl.grid = g < 1 ? 1 : g;
+        pl.iw = pick_iw(nact ? nact : 1, e.iw_fill);
+        if (e.tuned && e.K == 1 && nact) {
+            uint32_t bucket = 31u - (uint32_t)__clz(nact);
+            if (e.tune[bucket]) pl.grid = min(e.tune[bucket], e.grid);
+            if (e.tune[32 + bucket]) pl.iw = e.tune[32 + bucket];
+        }
+        Decomp d = decompose(nact, pl.iw, ntiles(e, c), pl.grid, e.max_split);
+        pl.S = d.S;
+        pl.tpc = d.tiles_per_chunk;
+        pl.n_it = d.n_it;
+        pl.rows = d.rows;
+        pl.lanes = finish_lanes(d.S);
+    }
+    return pl;
+}
+
+// Plan of a single-cluster context, made by every workgroup of the update
+// kernel from the active count (so the device step needs no k_plan
+// launch); workgroup 0 publishes it for the finish pass.
+__device__ __forceinline__ ClPlan plan_single(const Ens &e, uint32_t exact,
+                                              unsigned long long &items) {
+    const uint32_t nact = e.cs[0].nact;
+    const ClPlan pl = plan_cluster(e, 0, nact, (double)nact * (double)e.cnt[0], exact);
+    items = nact ? (unsigned long long)pl.n_it * pl.S : 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t per = BS / pl.lanes;
+        const unsigned long long units = (nact + per - 1) / per;
+        e.plan[0] = pl;
+        e.items_pre[0] = e.fin_pre[0] = e.part_pre[0] = 0ull;
+        e.items_pre[1] = items;
+        e.fin_pre[1] = units;
+        e.part_pre[1] = (nact && pl.S > 1) ? 6ull * pl.S * pl.rows : 0ull;
+        e.ctl->items = items;
+        e.ctl->fin_units = units;
+    }
+    return pl;
+}
+
 __device__ void plan_all(const Ens &e, uint32_t exact) {
     __shared__ double wsum[BS / 64];
     double w = 0.0;
@@ -189,37 +245,7 @@ __device__ void plan_all(const Ens &e, uint32_t exact) {
     __syncthreads();
     double W = 0.0;
     for (uint32_t k = 0; k < BS / 64; ++k) W += wsum[k];
-    for (uint32_t c = threadIdx.x; c < e.K; c += BS) {
-        uint32_t nact = e.cs[c].nact;
-        ClPlan pl;
-        pl.pad = 0;
-        if (exact) {
-            pl.iw = BS;
-            pl.S = 1;
-            pl.tpc = ntiles(e, c);
-            pl.n_it = (nact + BS - 1) / BS;
-            pl.rows = pl.n_it * BS;
-            pl.lanes = 1;
-            pl.grid = e.grid;
-        } else {
-            double share = W > 0.0 ? (double)nact * (double)e.cnt[c] / W : 0.0;
-            uint32_t g = (uint32_t)(share * e.grid);
-            pl.grid = g < 1 ? 1 : g;
-            pl.iw = pick_iw(nact ? nact : 1, e.iw_fill);
-            if (e.tuned && e.K == 1 && nact) {
-                uint32_t bucket = 31u - (uint32_t)__clz(nact);
-                if (e.tune[bucket]) pl.grid = min(e.tune[bucket], e.grid);
-                if (e.tune[32 + bucket]) pl.iw = e.tune[32 + bucket];
-            }
-            Decomp d = decompose(nact, pl.iw, ntiles(e, c), pl.grid, e.max_split);
-            pl.S = d.S;
-            pl.tpc = d.tiles_per_chunk;
-            pl.n_it = d.n_it;
-            pl.rows = d.rows;
-            pl.lanes = finish_lanes(d.S);
-        }
-        e.plan[c] = pl;
-    }
+    for (uint32_t c = threadIdx.x; c < e.K; c += BS) e.plan[c] = plan_cluster(e, c, e.cs[c].nact, W, exact);
     __syncthreads();
     // Exclusive prefix sums of work items, finish units and partial doubles
     // over the clusters, BS clusters at a time.
@@ -516,10 +542,15 @@ __global__ __launch_bounds__(BS) void k_update_exact(Soa s, Ens e, const uint32_
                                                     double e2) {
     __shared__ double4 sp[JT];
     __shared__ double4 sv[JT];
-    const unsigned long long total = e.ctl->items;
+    const bool single = e.K == 1;
+    unsigned long long total = 0;
+    if (single)
+        (void)plan_single(e, 1u, total);
+    else
+        total = e.ctl->items;
     for (unsigned long long item = blockIdx.x; item < total; item += gridDim.x) {
-        const uint32_t c = find_cluster(e.items_pre, e.K, item);
-        const uint32_t it = (uint32_t)(item - e.items_pre[c]);
+        const uint32_t c = single ? 0u : find_cluster(e.items_pre, e.K, item);
+        const uint32_t it = (uint32_t)(item - (single ? 0ull : e.items_pre[c]));
         const uint32_t nact = e.cs[c].nact;
         const uint32_t *mv = move + e.off[c];
         const uint32_t j_begin = e.off[c], j_end = e.off[c] + e.cnt[c];
@@ -624,18 +655,21 @@ __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const u
     __shared__ double4 lB[2 * JT];
     static_assert(BS * 6 * sizeof(double) <= sizeof(lA), "red overlays lA");
     double *red = reinterpret_cast<double *>(lA);
-    const unsigned long long total = e.ctl->items;
+    const bool single = e.K == 1;
+    unsigned long long total = 0;
+    const ClPlan pl0 = single ? plan_single(e, 0u, total) : ClPlan{};
+    if (!single) total = e.ctl->items;
     for (unsigned long long item = blockIdx.x; item < total; item += gridDim.x) {
-        const uint32_t c = find_cluster(e.items_pre, e.K, item);
-        const ClPlan pl = e.plan[c];
-        const uint32_t li = (uint32_t)(item - e.items_pre[c]);
+        const uint32_t c = single ? 0u : find_cluster(e.items_pre, e.K, item);
+        const ClPlan pl = single ? pl0 : e.plan[c];
+        const uint32_t li = (uint32_t)(item - (single ? 0ull : e.items_pre[c]));
         const uint32_t it = li / pl.S;
         const uint32_t sc = li - it * pl.S;
         const uint32_t nact = e.cs[c].nact;
         const uint32_t jt_begin = e.off[c] / JT;
         const uint32_t jt_end = jt_begin + ntiles(e, c);
         const uint32_t *mv = move + e.off[c];
-        double *part = partial + e.part_pre[c];
+        double *part = partial + (single ? 0ull : e.part_pre[c]);
         switch (pl.iw) {
             case 4:
                 update_item<4, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, lA, lB, red);
@@ -1271,7 +1305,8 @@ int enqueue_step(h4h_ctx *ctx, bool events) {
     if ((rc = launch_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD | PS_DEVTIME | PS_DIRTY,
                             0.0)))
         return rc;
-    if ((rc = launch_plan(ctx))) return rc;
+    // (a single-system context's update kernel makes its own plan)
+    if (ctx->K > 1 && (rc = launch_plan(ctx))) return rc;
     if (events) H4H_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     if ((rc = launch_update(ctx))) return rc;
     if (events) H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
