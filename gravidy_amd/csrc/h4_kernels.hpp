@@ -2,8 +2,9 @@
 //
 // State lives in HBM as structure-of-arrays (one double array per
 // component) so every kernel's per-particle loads are coalesced 8-byte
-// lanes.  The pairwise kernels stage j particles through LDS in tiles of
-// JT and read them back as wave-uniform (broadcast) LDS loads.
+// lanes; the predictor is kept as two double4 arrays in the layout of an
+// LDS j tile.  The pairwise kernels stage j particles through LDS in tiles
+// of JT and read them back as wave-uniform (broadcast) LDS loads.
 //
 // Rounding rules.  Predictor, corrector, timestep, init_dt and energy are
 // compiled with FP contraction off and use IEEE division and sqrt, so each

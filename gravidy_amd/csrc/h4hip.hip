@@ -143,9 +143,9 @@ __global__ __launch_bounds__(BS) void k_soa_to_scalar(const double *src, double 
     if (local < e.cnt[c]) dst[e.hoff[c] + local] = src[i];
 }
 
-// Predictor at t_i (Hermite4::init_data, Hermite4.cpp:194-240): p = r, v;
-// pm = (float)m.  Padding slots get zero mass far away and never become
-// active (t = 0, dt = +inf).
+// Predictor at t_i (Hermite4::init_data, Hermite4.cpp:194-240): position
+// and velocity as corrected, mass rounded to float.  Padding slots get zero
+// mass far away and never become active (t = 0, dt = +inf).
 __global__ __launch_bounds__(BS) void k_init_predictor(Soa s, Ens e) {
     uint32_t i = blockIdx.x * BS + threadIdx.x;
     if (i >= e.nblk * JT) return;
@@ -741,7 +741,7 @@ __device__ __forceinline__ double normalize_dt(double new_dt, double old_dt,
 // Hermite4CPU::correction_pos_vel (Hermite4CPU.cpp:370-411) with
 // NbodyUtils::get_timestep_normal (NbodyUtils.cpp:337-365) for particle i
 // at block time itime, given its new forces f[6] (old forces in s.old,
-// predictor in s.p).
+// predictor in s.pp / s.pv).  Returns the new dt.
 __device__ __forceinline__ void correct_one(const Soa &s, uint32_t i, const double f[6],
                                             double itime, float eta, double dt_min,
                                             double dt_max) {
