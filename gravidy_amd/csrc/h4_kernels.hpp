@@ -220,17 +220,25 @@ __device__ __forceinline__ void pair_exact(const double pi[6], double xj,
     fo[5] += (vz * mr3inv - (3 * rv) * rz * mr5inv);
 }
 
-// 1/sqrt(x) for x > 0: hardware estimate y0 (v_rsq_f64), then one
-// third-order step of the binomial series (1 - e)^(-1/2) = 1 + e/2 +
-// 3e^2/8 + ..., e = 1 - x*y0^2, which triples the number of correct bits.
-__device__ __forceinline__ double rsqrt_fp64(double x) {
+// The FP64 pair loop works with s = sqrt(3) / r and q = s^2 = 3 / r^2:
+// the jerk coefficient 3 (r.v) / r^2 is then (r.v) q, with no multiply by
+// 3, and m s q = 3 sqrt(3) m / r^3, so every pair term it sums is
+// 3 sqrt(3) times the true one.  The sums are multiplied by
+// PAIR_SCALE = 1 / (3 sqrt 3) once, when the update writes them out.
+constexpr double PAIR_SCALE = 0.19245008972987526;
+
+// sqrt(3) / sqrt(x) for x > 0: hardware estimate y = (1 + d) / sqrt(x)
+// (v_rsq_f64; |d| < 2^-24 over the FP64 range, measured by valu-probe),
+// then one Newton step sqrt(3) y (1 + e/2), e = 1 - x y^2, which leaves a
+// relative error of about -1.5 d^2, below 2^-47.
+__device__ __forceinline__ double rsqrt3_fp64(double x) {
     double y = __builtin_amdgcn_rsq(x);
     double e = fma(-x, y * y, 1.0);
-    double p = fma(e, 0.375, 0.5);
-    return fma(y * e, p, y);
+    return y * fma(e, 0.8660254037844386, 1.7320508075688772);  // sqrt3 / 2, sqrt3
 }
 
-// Same pair as pair_exact with FMA contraction and rsqrt_fp64:
+// Same pair as pair_exact with FMA contraction and rsqrt3_fp64, scaled by
+// 3 sqrt(3) (see PAIR_SCALE):
 //   a  += m r / |r|^3
 //   j  += m/|r|^3 (v - 3 (r.v)/|r|^2 r)
 __device__ __forceinline__ void pair_fp64(const double pi[6], double xj,
@@ -244,11 +252,11 @@ __device__ __forceinline__ void pair_fp64(const double pi[6], double xj,
     double vy = vyj - pi[4];
     double vz = vzj - pi[5];
     double r2 = fma(rx, rx, fma(ry, ry, fma(rz, rz, e2)));
-    double rinv = rsqrt_fp64(r2);
-    double rinv2 = rinv * rinv;
-    double mr3 = (mj * rinv) * rinv2;
+    double s = rsqrt3_fp64(r2);  // sqrt(3) / r
+    double q = s * s;            // 3 / r^2
+    double mr3 = (mj * s) * q;   // 3 sqrt(3) m / r^3
     double rv = fma(rx, vx, fma(ry, vy, rz * vz));
-    double c = (3.0 * rv) * rinv2;
+    double c = rv * q;           // 3 (r.v) / r^2
     fo[0] = fma(rx, mr3, fo[0]);
     fo[1] = fma(ry, mr3, fo[1]);
     fo[2] = fma(rz, mr3, fo[2]);
@@ -258,7 +266,8 @@ __device__ __forceinline__ void pair_fp64(const double pi[6], double xj,
 }
 
 // 1/sqrt(x) in FP32: v_rsq_f32 estimate and one Newton step
-// y (1 + e/2), e = 1 - x y^2.
+// y (1 + e/2), e = 1 - x y^2.  (No sqrt(3) scaling here: the FP32 value of
+// sqrt(3) is 1.8e-8 off, a bias every pair term would share.)
 __device__ __forceinline__ float rsqrt_fp32(float x) {
     float y = __builtin_amdgcn_rsqf(x);
     float e = fmaf(-x, y * y, 1.0f);

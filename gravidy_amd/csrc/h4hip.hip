@@ -636,11 +636,12 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
     }
     reduce_subgroups<IW>(fo, red);
     if (threadIdx.x < IW && act) {
+        constexpr double scale = MIXED ? 1.0 : PAIR_SCALE;  // pair_fp64's 3 sqrt(3)
         if (pl.S == 1) {
-            for (int q = 0; q < 6; ++q) s.f[q][i] = fo[q];
+            for (int q = 0; q < 6; ++q) s.f[q][i] = fo[q] * scale;
         } else {
             for (int q = 0; q < 6; ++q)
-                partial[((size_t)q * pl.S + sc) * pl.rows + k] = fo[q];
+                partial[((size_t)q * pl.S + sc) * pl.rows + k] = fo[q] * scale;
         }
     }
 }
@@ -689,7 +690,8 @@ __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const u
 
 // The four lane-group variants in one kernel would take 144 VGPRs (3 waves
 // per SIMD); each alone needs <= 126, so the FP64 kernel is held to 128
-// (4 waves per SIMD, no spills).  The mixed kernel fits in 108 by itself.
+// (4 waves per SIMD; a few per-item values spill to scratch, none inside
+// the tile or pair loops).  The mixed kernel fits in 120 by itself.
 __global__ __launch_bounds__(BS, 4) void k_update_fp64(Soa s, Ens e, const uint32_t *move,
                                                       double *partial, double e2) {
     update_items<false>(s, e, move, partial, e2);

@@ -3,11 +3,16 @@
 // dependency chains per lane, 2048 workgroups of 256 threads (8 waves per
 // SIMD on a 256-CU part), all in registers; the report is nanoseconds per
 // chain step per SIMD, i.e. per wave64 instruction group of the listed
-// kind.  Used to explain the mixed-precision pair's cost; not part of the
-// library.
+// kind.  Second part: the relative error of the v_rsq_f64 and v_rsq_f32
+// estimates that seed the pair kernels' 1/r (one Newton step has to take
+// them to the FP64 path's accuracy).  Used to explain and bound the pair
+// kernels; not part of the library.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
+#include <random>
+#include <vector>
 
 namespace {
 
@@ -71,6 +76,56 @@ float time_op(double *od, float *of, hipEvent_t e0, hipEvent_t e1) {
     return ms;
 }
 
+__global__ __launch_bounds__(256) void rsq_seeds(const double *x, double *y, float *yf, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        y[i] = __builtin_amdgcn_rsq(x[i]);
+        yf[i] = __builtin_amdgcn_rsqf((float)x[i]);
+    }
+}
+
+// Largest |y sqrt(x) - 1| of the hardware estimates over 2^22 inputs: a
+// uniform sweep of [1, 4) (the estimate's period in x is a factor of 4)
+// and log-uniform values over 2^-60 .. 2^60; the reference is long double.
+int rsq_accuracy() {
+    const int n = 1 << 22;
+    std::vector<double> x(n), y(n);
+    std::vector<float> yf(n);
+    std::mt19937_64 g(12345);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    for (int i = 0; i < n / 2; ++i) x[i] = 1.0 + 3.0 * (i + 0.5) / (n / 2);
+    for (int i = n / 2; i < n; ++i) x[i] = std::ldexp(1.0 + u(g), (int)(u(g) * 120) - 60);
+    double *dx = nullptr, *dy = nullptr;
+    float *dyf = nullptr;
+    if (hipMalloc(&dx, n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dy, n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dyf, n * sizeof(float)) != hipSuccess ||
+        hipMemcpy(dx, x.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        std::fprintf(stderr, "valu-probe: HIP setup failed\n");
+        return 1;
+    }
+    hipLaunchKernelGGL(rsq_seeds, dim3(n / 256), dim3(256), 0, 0, dx, dy, dyf, n);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(y.data(), dy, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(yf.data(), dyf, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+        std::fprintf(stderr, "valu-probe: rsq run failed\n");
+        return 1;
+    }
+    long double e64 = 0, e32 = 0;
+    for (int i = 0; i < n; ++i) {
+        const long double xd = x[i], xf = (float)x[i];
+        e64 = std::fmax(e64, std::fabs((long double)y[i] * std::sqrt(xd) - 1.0L));
+        e32 = std::fmax(e32, std::fabs((long double)yf[i] * std::sqrt(xf) - 1.0L));
+    }
+    std::printf("rsq estimate, max relative error over %d inputs: v_rsq_f64 %.3Le (2^%.2f), "
+                "v_rsq_f32 %.3Le (2^%.2f)\n",
+                n, e64, (double)std::log2(e64), e32, (double)std::log2(e32));
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    (void)hipFree(dyf);
+    return 0;
+}
+
 }  // namespace
 
 int main() {
@@ -114,5 +169,5 @@ int main() {
     }
     (void)hipFree(od);
     (void)hipFree(of);
-    return 0;
+    return rsq_accuracy();
 }

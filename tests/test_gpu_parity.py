@@ -392,6 +392,32 @@ def test_update_mixed_within_tolerance(oracle, n, nact):
     check_mixed(f, fo, move, f"update n={n} nact={nact} mixed")
 
 
+def test_set_precision_between_updates_matches_fresh_contexts():
+    """h4h_set_precision re-encodes the predictor's pair weights (a double
+    for the FP64 kernel, float bits for the mixed one): after every switch
+    the update is bit-identical to that of a context created with the new
+    precision."""
+    n, nact = 4096, 700
+    rng = np.random.default_rng(11)
+    r, v = ic.plummer(n, 77)
+    move = np.sort(rng.choice(n, size=nact, replace=False)).astype(np.uint32)
+
+    def fresh(prec):
+        with Hermite4HIP(r, v, E2, precision=prec) as h:
+            h.set_active(move)
+            h.update_acc_jrk(nact)
+            return h.download()["f"]
+
+    want = {p: fresh(p) for p in ("fp64", "mixed", "exact")}
+    with Hermite4HIP(r, v, E2, precision="fp64") as h:
+        h.set_active(move)
+        for prec in ("mixed", "fp64", "exact", "mixed", "exact", "fp64"):
+            h.set_precision(prec)
+            h.update_acc_jrk(nact)
+            f = h.download()["f"]
+            assert bits_equal(f[move], want[prec][move]), prec
+
+
 def test_plummer1024_run_mixed_energy_drift(oracle):
     """Plummer N=1024 to t=1 with the mixed kernel: the cumulative energy
     error at t_end stays within 2x of the FP64 path's."""
