@@ -823,14 +823,34 @@ __global__ __launch_bounds__(BS) void k_finish(Soa s, Ens e, const uint32_t *mov
         if (pl.S == 1) {
             for (int q = 0; q < 6; ++q) f[q] = valid ? s.f[q][i] : 0.0;
         } else {
-            const double *part = partial + e.part_pre[c];
+            // Lane sl adds chunks sl, sl + L, ... in that order.  The loads
+            // of a round of U chunks of all six components are issued
+            // together (a few memory latencies per round instead of one per
+            // chunk; S <= 1024 takes two rounds at L = 64).
+            constexpr uint32_t U = 8;
+            const double *part = partial + e.part_pre[c] + k;
+            const size_t qs = (size_t)pl.S * pl.rows;
+            double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if (valid) {
+                for (uint32_t s0 = sl; s0 < pl.S; s0 += U * L) {
+                    double v[6][U];
+#pragma unroll
+                    for (int q = 0; q < 6; ++q)
+#pragma unroll
+                        for (uint32_t u = 0; u < U; ++u) {
+                            const uint32_t sc = s0 + u * L;
+                            v[q][u] = sc < pl.S ? part[q * qs + (size_t)sc * pl.rows] : 0.0;
+                        }
+#pragma unroll
+                    for (int q = 0; q < 6; ++q)
+#pragma unroll
+                        for (uint32_t u = 0; u < U; ++u)
+                            if (s0 + u * L < pl.S) acc[q] += v[q][u];
+                }
+            }
             for (int q = 0; q < 6; ++q) {
-                double acc = 0.0;
-                if (valid)
-                    for (uint32_t sc = sl; sc < pl.S; sc += L)
-                        acc += part[((size_t)q * pl.S + sc) * pl.rows + k];
-                for (uint32_t o = 1; o < L; o <<= 1) acc += __shfl_xor(acc, o);
-                f[q] = acc;
+                for (uint32_t o = 1; o < L; o <<= 1) acc[q] += __shfl_xor(acc[q], o);
+                f[q] = acc[q];
             }
         }
         if (valid && sl == 0) {
