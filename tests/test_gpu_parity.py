@@ -292,6 +292,18 @@ def test_energy_matches(oracle, n):
     assert ep == pytest.approx(epo, rel=1e-13, abs=1e-300)
 
 
+def test_energy_of_coincident_particles_is_minus_infinity(oracle):
+    """The energy is unsoftened: two particles at one position give -inf
+    in the reference's arithmetic (division by sqrt(0)) and on the GPU."""
+    r = np.array([[0.5, 0.25, -1.0, 0.5], [0.5, 0.25, -1.0, 0.25], [1.0, 0.0, 0.0, 0.25]])
+    v = np.zeros((3, 4))
+    with Hermite4HIP(r, v, E2) as h:
+        ek, ep = h.get_energy()
+    eko, epo = oracle.get_energy(r, v)
+    assert epo == -np.inf and ep == -np.inf
+    assert ek == eko == 0.0
+
+
 # --- whole runs -------------------------------------------------------------
 
 def test_plummer1024_run_exact_is_bitwise(oracle):
