@@ -42,8 +42,9 @@ typedef struct h4h_ctx h4h_ctx;
 #define H4H_ESTATE       -4   /* call out of order (e.g. no upload yet)  */
 #define H4H_ENODEV       -5   /* no HIP device                           */
 
-/* Pairwise-kernel precision.  The predictor, corrector, block-timestep
- * selection and energy are always FP64 with the reference's rounding. */
+/* Pairwise-kernel precision.  The predictor, corrector and block-timestep
+ * selection are always FP64 with the reference's rounding; the energy is
+ * always FP64 and within 1e-13 of the reference's. */
 #define H4H_FP64        0  /* FP64, rsqrt + refinement, FMA, j-split     */
 #define H4H_FP64_EXACT  1  /* FP64 bit-identical to Hermite4CPU:
                               1.0/sqrt, no FMA, ascending j, no j-split   */
@@ -244,7 +245,9 @@ int h4h_ensemble_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_ste
 /* Kinetic and potential energy of every cluster (nclusters each). */
 int h4h_ensemble_energy(h4h_ctx *ctx, double *ekin, double *epot);
 
-/* Kinetic and (unsoftened) potential energy of the corrected state.
+/* Kinetic and (unsoftened) potential energy of the corrected state, in
+ * FP64, within 1e-13 relative of the reference's sums (1/r from a refined
+ * hardware rsqrt, compensated sums; coincident particles give -inf).
  * Replaces: NbodyUtils::get_energy (utils/NbodyUtils.cpp:427-529) and
  * Hermite4GPU::get_energy_gpu + k_energy (gpu/Hermite4GPU.cu:540-592,
  * Hermite4GPU_kernels.cu:250-288). */
