@@ -1062,13 +1062,18 @@ __global__ void k_set_times_from_tmin(Ens e, double itime) {
 // -inf as in the reference.
 // ---------------------------------------------------------------------------
 
+// Neumaier-compensated sum += x.  Once the sum is infinite (coincident
+// particles in the unsoftened energy) the compensation is left alone, so
+// the result is the infinity a plain sum gives rather than inf - inf.
 __device__ __forceinline__ void neumaier(double &sum, double &comp, double x) {
 #pragma clang fp contract(off)
     double t = sum + x;
-    if (fabs(sum) >= fabs(x))
-        comp += (sum - t) + x;
-    else
-        comp += (x - t) + sum;
+    if (isfinite(t)) {
+        if (fabs(sum) >= fabs(x))
+            comp += (sum - t) + x;
+        else
+            comp += (x - t) + sum;
+    }
     sum = t;
 }
 
@@ -1442,14 +1447,17 @@ int energy_of(h4h_ctx *ctx, uint32_t c, double *ekin, double *epot) {
     H4H_HIP(hipMemcpyAsync(h.data(), ctx->eout, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                            ctx->stream));
     H4H_HIP(hipStreamSynchronize(ctx->stream));
-    // Neumaier sum of the per-workgroup (sum, compensation) pairs.
+    // Neumaier sum of the per-workgroup (sum, compensation) pairs (an
+    // infinite sum leaves the compensation alone, as on the device).
     double sp = 0.0, cp = 0.0, sk = 0.0, ck = 0.0;
     auto add = [](double &s, double &c2, double x) {
         double t = s + x;
-        if (std::fabs(s) >= std::fabs(x))
-            c2 += (s - t) + x;
-        else
-            c2 += (x - t) + s;
+        if (std::isfinite(t)) {
+            if (std::fabs(s) >= std::fabs(x))
+                c2 += (s - t) + x;
+            else
+                c2 += (x - t) + s;
+        }
         s = t;
     };
     for (uint32_t b = 0; b < ENERGY_GRID; ++b) {
