@@ -70,6 +70,20 @@ def test_driver_plummer1024_outputs(oracle, tmp_path, precision):
     assert len(first[1]) == 6 + 7 * 15
 
 
+def test_driver_plummer1024_mixed(oracle, tmp_path):
+    """--precision mixed through the driver: the run reaches t = 1 with the
+    full output table, energies within 1e-6 of the oracle's FP64 run, and
+    a cumulative energy error within 2x of it."""
+    r, v, out = _run(tmp_path, "mixed")
+    ref = oracle.integrate(r, v, softening_e2(1e-4), 1.0)
+    rows = [ln.split() for ln in open(out + ".log").read().splitlines()[1:]]
+    assert [float(x[1]) for x in rows] == [0.125 * k for k in range(9)]
+    energies = np.array([float(x[4]) for x in rows])
+    np.testing.assert_allclose(energies, ref["energy"], rtol=1e-6)
+    cum_ref = abs((ref["energy"][-1] - ref["energy"][0]) / ref["energy"][0])
+    assert float(rows[-1][6]) <= 2.0 * cum_ref
+
+
 def test_driver_rejects_bad_input(tmp_path):
     bad = tmp_path / "bad.txt"
     bad.write_text("0 0.5 1 0 0 0 0\n")

@@ -561,3 +561,22 @@ def test_ensemble_fp64_energy_errors_match_oracle(oracle):
         print(f"cluster {c} n={sizes[c]}: CumE {cum:.4e} oracle {cum_ref:.4e} "
               f"steps {res['steps'][c]}/{len(ref['step_time'])}")
         assert abs(cum - cum_ref) <= 0.05 * cum_ref + 1e-15
+
+
+def test_ensemble_mixed_energy_drift_within_2x_fp64():
+    """Mixed kernel in an ensemble (5 clusters of 1k-4k to t=0.25): each
+    cluster's cumulative energy error is within 2x its FP64 one, and every
+    cluster reaches t_end."""
+    from gravidy_amd import EnsembleHIP
+    sizes = [1024, 1500, 2048, 3000, 4096]
+    systems = [ic.plummer(n, 900 + n) for n in sizes]
+    cum = {}
+    for precision in ("fp64", "mixed"):
+        with EnsembleHIP(systems, E2, precision=precision) as ens:
+            res = ens.integration(0.25)
+        e = res["energy"]
+        assert list(res["energy_time"]) == [0.0, 0.125, 0.25]
+        cum[precision] = np.abs((e[-1] - e[0]) / e[0])
+    for c, n in enumerate(sizes):
+        print(f"cluster {c} n={n}: CumE fp64 {cum['fp64'][c]:.4e} mixed {cum['mixed'][c]:.4e}")
+        assert cum["mixed"][c] <= 2.0 * cum["fp64"][c] + 1e-12
