@@ -87,6 +87,8 @@ struct RunCtl {
     uint32_t pad[2];
     unsigned long long items;      // update work items this step
     unsigned long long fin_units;  // finish workgroup units this step
+    unsigned long long qnext;      // update work queue: items claimed beyond the
+                                   // first gridDim.x (zero between launches)
 };
 
 // Cluster tables and per-step scratch, by pointer (device memory).
@@ -109,6 +111,7 @@ struct Ens {
     uint32_t *hist_n;
     uint32_t hist_cap;
     uint32_t grid;           // persistent workgroups of the update kernel
+    uint32_t target;         // work items a plan aims at (grid x queue factor)
     uint32_t max_split;
     uint32_t iw_fill;        // pick_iw fill rule, sixteenths
     uint32_t tuned;          // 1: tune[] holds a table for single-cluster plans

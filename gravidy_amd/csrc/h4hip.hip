@@ -54,6 +54,7 @@ int fail(int code, const std::string &msg) {
 
 constexpr uint32_t DEFAULT_GRID = 2048;       // persistent workgroups
 constexpr uint32_t DEFAULT_MAX_SPLIT = 1024;  // largest j split
+constexpr uint32_t QUEUE_FACTOR = 4;          // plans aim at this many items per workgroup
 constexpr uint32_t ENERGY_GRID = 1024;
 
 // ---------------------------------------------------------------------------
@@ -195,12 +196,12 @@ __device__ __forceinline__ ClPlan plan_cluster(const Ens &e, uint32_t c, uint32_
         pl.grid = e.grid;
     } else {
         double share = W > 0.0 ? (double)nact * (double)e.cnt[c] / W : 0.0;
-        uint32_t g = (uint32_t)(share * e.grid);
+        uint32_t g = (uint32_t)(share * e.target);
         pl.grid = g < 1 ? 1 : g;
         pl.iw = pick_iw(nact ? nact : 1, e.iw_fill);
         if (e.tuned && e.K == 1 && nact) {
             uint32_t bucket = 31u - (uint32_t)__clz(nact);
-            if (e.tune[bucket]) pl.grid = min(e.tune[bucket], e.grid);
+            if (e.tune[bucket]) pl.grid = min(e.tune[bucket], e.target);
             if (e.tune[32 + bucket]) pl.iw = e.tune[32 + bucket];
         }
         Decomp d = decompose(nact, pl.iw, ntiles(e, c), pl.grid, e.max_split);
@@ -660,7 +661,12 @@ __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const u
     unsigned long long total = 0;
     const ClPlan pl0 = single ? plan_single(e, 0u, total) : ClPlan{};
     if (!single) total = e.ctl->items;
-    for (unsigned long long item = blockIdx.x; item < total; item += gridDim.x) {
+    // The first gridDim.x items go one to each workgroup; a workgroup that
+    // has finished one takes the next unclaimed item from the queue counter
+    // (zeroed again by the finish pass), so faster CUs and XCDs take more.
+    __shared__ unsigned long long claimed;
+    unsigned long long item = blockIdx.x;
+    while (item < total) {
         const uint32_t c = single ? 0u : find_cluster(e.items_pre, e.K, item);
         const ClPlan pl = single ? pl0 : e.plan[c];
         const uint32_t li = (uint32_t)(item - (single ? 0ull : e.items_pre[c]));
@@ -685,6 +691,11 @@ __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const u
                 update_item<256, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, lA, lB, red);
                 break;
         }
+        if (total <= gridDim.x) break;
+        __syncthreads();  // every lane has read the previous claim
+        if (threadIdx.x == 0) claimed = gridDim.x + atomicAdd(&e.ctl->qnext, 1ull);
+        __syncthreads();
+        item = claimed;
     }
 }
 
@@ -809,6 +820,7 @@ __global__ __launch_bounds__(BS) void k_finish(Soa s, Ens e, const uint32_t *mov
                                               const double *partial, uint32_t mode, float eta,
                                               double dt_min, double dt_max) {
     const unsigned long long total = e.ctl->fin_units;
+    if (blockIdx.x == 0 && threadIdx.x == 0) e.ctl->qnext = 0ull;  // the update is over
     for (unsigned long long u = blockIdx.x; u < total; u += gridDim.x) {
         const uint32_t c = find_cluster(e.fin_pre, e.K, u);
         const ClPlan pl = e.plan[c];
@@ -1269,14 +1281,18 @@ void drop_graph(h4h_ctx *ctx) {
 // (Re)sizes the j-split partials for the current grid and updates the
 // launch fields of the cluster table.
 int configure_launch(h4h_ctx *ctx) {
+    const uint32_t target = ctx->grid * QUEUE_FACTOR;
     uint32_t g = ctx->grid;
     if (ctx->K == 1)
-        g = std::min(g, max_items(ctx->cnt[0], ctx->grid, ctx->max_split, ctx->iw_fill));
+        g = std::min(g, max_items(ctx->cnt[0], target, ctx->max_split, ctx->iw_fill));
     ctx->e.grid = g;
+    ctx->e.target = target;
     ctx->e.max_split = ctx->max_split;
     ctx->e.iw_fill = ctx->iw_fill;
     drop_graph(ctx);
-    size_t need = (size_t)6 * 2 * BS * ((size_t)g + ctx->K);
+    // j-split partials: a cluster with share g_c of the target and S > 1
+    // chunks has n_it < g_c i tiles, so 6 S rows <= 6 * 2 g_c * BS doubles.
+    size_t need = (size_t)6 * 2 * BS * ((size_t)target + ctx->K);
     if (ctx->partial && ctx->partial_len >= need) return H4H_OK;
     if (ctx->partial) H4H_HIP(hipFree(ctx->partial));
     ctx->partial = nullptr;
