@@ -6,10 +6,11 @@
 // LDS j tile.  The pairwise kernels stage j particles through LDS in tiles
 // of JT and read them back as wave-uniform (broadcast) LDS loads.
 //
-// Rounding rules.  Predictor, corrector, timestep, init_dt and energy are
-// compiled with FP contraction off and use IEEE division and sqrt, so each
+// Rounding rules.  Predictor, corrector, timestep and init_dt are compiled
+// with FP contraction off and use IEEE division and sqrt, so each
 // operation rounds exactly as the reference's x86-64 build does
-// (Hermite4CPU.cpp:340-411, NbodyUtils.cpp:337-529, Hermite4.cpp:108-140).
+// (Hermite4CPU.cpp:340-411, NbodyUtils.cpp:337-424, Hermite4.cpp:108-140);
+// the energy sum keeps the reference's terms and rounds within 1e-13.
 // The FP64_EXACT pair kernel does the same for the pair terms and keeps the
 // ascending-j order of Hermite4CPU::update_acc_jrk.  The FP64 pair kernel
 // contracts to FMA, uses a refined v_rsq_f64 and may split the j range.
@@ -160,6 +161,28 @@ __host__ __device__ inline Decomp decompose(uint32_t nact, uint32_t iw,
     d.n_items = d.n_it * d.S;
     d.rows = d.n_it * iw;
     return d;
+}
+
+// Upper bound of the j-split partials (doubles) of one cluster of n
+// particles over every plan decompose() can make for a work-item target of
+// at most g: any active count, any lane-group width iw <= BS, any split
+// cap.  With S > 1 chunks, S <= ceil(g / n_it), so S n_it <= g + n_it - 1
+// and 6 S rows = 6 S n_it iw <= 6 (g iw + n + iw); also S <= n_jt and
+// rows <= n + iw, so 6 S rows <= 6 n_jt (n + iw).
+__host__ __device__ inline unsigned long long partial_bound(uint32_t n, uint32_t g) {
+    const unsigned long long iw = BS, n_jt = (n + JT - 1) / JT;
+    const unsigned long long a = (unsigned long long)g * iw + n + iw;
+    const unsigned long long b = n_jt * ((unsigned long long)n + iw);
+    return 6ull * (a < b ? a : b);
+}
+
+// The same for K clusters of total size n_total whose work-item targets
+// g_c = max(1, floor(share_c * g)) (shares summing to at most 1) add up to
+// at most g + K: 6 sum_c (g_c BS + n_c + BS) <= 6 (BS (g + 2K) + n_total).
+__host__ __device__ inline unsigned long long ensemble_partial_bound(uint32_t K,
+                                                                     unsigned long long n_total,
+                                                                     uint32_t g) {
+    return 6ull * ((unsigned long long)BS * ((unsigned long long)g + 2ull * K) + n_total);
 }
 
 // Lanes of the finish pass that share one particle's S chunk sums: each

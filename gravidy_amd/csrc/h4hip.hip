@@ -54,7 +54,7 @@ int fail(int code, const std::string &msg) {
 
 constexpr uint32_t DEFAULT_GRID = 2048;       // persistent workgroups
 constexpr uint32_t DEFAULT_MAX_SPLIT = 1024;  // largest j split
-constexpr uint32_t QUEUE_FACTOR = 4;          // plans aim at this many items per workgroup
+constexpr uint32_t QUEUE_FACTOR = 16;         // plans aim at this many items per workgroup
 constexpr uint32_t ENERGY_GRID = 1024;
 
 // ---------------------------------------------------------------------------
@@ -1290,9 +1290,18 @@ int configure_launch(h4h_ctx *ctx) {
     ctx->e.max_split = ctx->max_split;
     ctx->e.iw_fill = ctx->iw_fill;
     drop_graph(ctx);
-    // j-split partials: a cluster with share g_c of the target and S > 1
-    // chunks has n_it < g_c i tiles, so 6 S rows <= 6 * 2 g_c * BS doubles.
-    size_t need = (size_t)6 * 2 * BS * ((size_t)target + ctx->K);
+    // j-split partials for any plan of any step (partial_bound).
+    size_t need;
+    if (ctx->K == 1) {
+        need = partial_bound(ctx->cnt[0], target);
+    } else {
+        unsigned long long n_total = 0, per_cluster = 0;
+        for (uint32_t c = 0; c < ctx->K; ++c) {
+            n_total += ctx->cnt[c];
+            per_cluster += partial_bound(ctx->cnt[c], target);
+        }
+        need = std::min(per_cluster, ensemble_partial_bound(ctx->K, n_total, target));
+    }
     if (ctx->partial && ctx->partial_len >= need) return H4H_OK;
     if (ctx->partial) H4H_HIP(hipFree(ctx->partial));
     ctx->partial = nullptr;
