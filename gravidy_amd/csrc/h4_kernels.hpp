@@ -73,7 +73,7 @@ struct ClPlan {
     uint32_t tpc;       // JT tiles per chunk
     uint32_t n_it;      // i tiles
     uint32_t rows;      // n_it * iw
-    uint32_t lanes;     // finish lanes per particle (finish_lanes(S))
+    uint32_t lanes;     // finish lanes per particle (finish_lanes(S, nact))
     uint32_t grid;      // workgroup share given to decompose()
     uint32_t pad;
 };
@@ -112,7 +112,6 @@ struct Ens {
     uint32_t *hist_n;
     uint32_t hist_cap;
     uint32_t grid;           // persistent workgroups of the update kernel
-    uint32_t target;         // work items a plan aims at (grid x queue factor)
     uint32_t max_split;
     uint32_t iw_fill;        // pick_iw fill rule, sixteenths
     uint32_t tuned;          // 1: tune[] holds a table for single-cluster plans
@@ -133,9 +132,16 @@ __host__ __device__ inline double dkey_inv(unsigned long long k) {
 // Work split of the pairwise update.  A workgroup handles IW active
 // particles ("i slots") at a time; its BS lanes are IW slots x BS/IW j
 // sub-groups.  The nact active particles form n_it = ceil(nact/IW) i tiles
-// and the j range (n_jt tiles of JT) is cut into S chunks so that
-// n_it * S work items keep `grid` workgroups busy when nact is small.
-// A chunk holds at least 64/IW tiles so one item is >= 16K pairs.
+// and the j range (n_jt tiles of JT) is cut into S chunks, giving n_it * S
+// work items.  S is large enough that every one of `grid` workgroups gets
+// an item when nact is small, and beyond that up to QUEUE_FACTOR items per
+// workgroup (taken from the update kernel's work queue, which evens out
+// faster and slower CUs and XCDs) as long as a chunk keeps CHUNK_TILES
+// tiles (more chunks would mostly add partial sums to write and read).  A
+// chunk holds at least 64/IW tiles so one item is >= 16K pairs.
+constexpr uint32_t QUEUE_FACTOR = 16;
+constexpr uint32_t CHUNK_TILES = 16;
+
 struct Decomp {
     uint32_t n_it;             // i tiles
     uint32_t S;                // j chunks
@@ -151,7 +157,14 @@ __host__ __device__ inline Decomp decompose(uint32_t nact, uint32_t iw,
     d.n_it = (nact + iw - 1) / iw;
     uint32_t min_tiles = iw >= 64 ? 1u : 64u / iw;
     uint32_t S = 1;
-    if (d.n_it > 0 && d.n_it < grid) S = (grid + d.n_it - 1) / d.n_it;
+    const uint32_t target = grid * QUEUE_FACTOR;
+    if (d.n_it > 0 && d.n_it < target) {
+        const uint32_t s_fill = d.n_it < grid ? (grid + d.n_it - 1) / d.n_it : 1u;
+        const uint32_t s_queue = (target + d.n_it - 1) / d.n_it;
+        const uint32_t s_tiles = n_jt / CHUNK_TILES;
+        const uint32_t s_most = s_fill > s_tiles ? s_fill : s_tiles;
+        S = s_queue < s_most ? s_queue : s_most;
+    }
     if (S > max_split) S = max_split;
     uint32_t s_cap = (n_jt + min_tiles - 1) / min_tiles;
     if (S > s_cap) S = s_cap;
@@ -164,9 +177,10 @@ __host__ __device__ inline Decomp decompose(uint32_t nact, uint32_t iw,
 }
 
 // Upper bound of the j-split partials (doubles) of one cluster of n
-// particles over every plan decompose() can make for a work-item target of
-// at most g: any active count, any lane-group width iw <= BS, any split
-// cap.  With S > 1 chunks, S <= ceil(g / n_it), so S n_it <= g + n_it - 1
+// particles over every plan decompose() can make for a workgroup share of
+// at most g / QUEUE_FACTOR: any active count, any lane-group width
+// iw <= BS, any split cap.  With S > 1 chunks, S <= ceil(g / n_it) (the
+// queue target), so S n_it <= g + n_it - 1
 // and 6 S rows = 6 S n_it iw <= 6 (g iw + n + iw); also S <= n_jt and
 // rows <= n + iw, so 6 S rows <= 6 n_jt (n + iw).
 __host__ __device__ inline unsigned long long partial_bound(uint32_t n, uint32_t g) {
@@ -176,19 +190,26 @@ __host__ __device__ inline unsigned long long partial_bound(uint32_t n, uint32_t
     return 6ull * (a < b ? a : b);
 }
 
-// The same for K clusters of total size n_total whose work-item targets
-// g_c = max(1, floor(share_c * g)) (shares summing to at most 1) add up to
-// at most g + K: 6 sum_c (g_c BS + n_c + BS) <= 6 (BS (g + 2K) + n_total).
+// The same for K clusters of total size n_total whose workgroup shares
+// w_c = max(1, floor(share_c * w)) (shares summing to at most 1) give
+// queue targets g_c = QUEUE_FACTOR w_c adding up to at most g + QUEUE_FACTOR K,
+// g = QUEUE_FACTOR w: 6 sum_c (g_c BS + n_c + BS) <= 6 (BS (g + (QUEUE_FACTOR + 1) K) + n_total).
 __host__ __device__ inline unsigned long long ensemble_partial_bound(uint32_t K,
                                                                      unsigned long long n_total,
                                                                      uint32_t g) {
-    return 6ull * ((unsigned long long)BS * ((unsigned long long)g + 2ull * K) + n_total);
+    return 6ull * ((unsigned long long)BS * ((unsigned long long)g + (QUEUE_FACTOR + 1ull) * K) +
+                   n_total);
 }
 
 // Lanes of the finish pass that share one particle's S chunk sums: each
-// lane adds at most 8 of them (1, 4, 16 or 64 lanes).
-__host__ __device__ inline uint32_t finish_lanes(uint32_t S) {
-    return S <= 8 ? 1u : S <= 32 ? 4u : S <= 128 ? 16u : 64u;
+// lane adds at most 8 of them (1, 4, 16 or 64 lanes), except that many
+// active particles (nact >= 8192, enough to fill the part) take at most 4
+// lanes each: a wave then reads whole 128-byte lines of consecutive
+// particles' partials rather than one double per line.
+__host__ __device__ inline uint32_t finish_lanes(uint32_t S, uint32_t nact) {
+    if (S <= 8) return 1u;
+    if (S <= 32 || nact >= 8192) return 4u;
+    return S <= 128 ? 16u : 64u;
 }
 
 // Lane-group width for nact active particles: the widest of 256/64/16/4

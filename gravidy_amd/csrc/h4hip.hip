@@ -54,7 +54,6 @@ int fail(int code, const std::string &msg) {
 
 constexpr uint32_t DEFAULT_GRID = 2048;       // persistent workgroups
 constexpr uint32_t DEFAULT_MAX_SPLIT = 1024;  // largest j split
-constexpr uint32_t QUEUE_FACTOR = 16;         // plans aim at this many items per workgroup
 constexpr uint32_t ENERGY_GRID = 1024;
 
 // ---------------------------------------------------------------------------
@@ -196,12 +195,12 @@ __device__ __forceinline__ ClPlan plan_cluster(const Ens &e, uint32_t c, uint32_
         pl.grid = e.grid;
     } else {
         double share = W > 0.0 ? (double)nact * (double)e.cnt[c] / W : 0.0;
-        uint32_t g = (uint32_t)(share * e.target);
+        uint32_t g = (uint32_t)(share * e.grid);
         pl.grid = g < 1 ? 1 : g;
         pl.iw = pick_iw(nact ? nact : 1, e.iw_fill);
         if (e.tuned && e.K == 1 && nact) {
             uint32_t bucket = 31u - (uint32_t)__clz(nact);
-            if (e.tune[bucket]) pl.grid = min(e.tune[bucket], e.target);
+            if (e.tune[bucket]) pl.grid = min(e.tune[bucket], e.grid);
             if (e.tune[32 + bucket]) pl.iw = e.tune[32 + bucket];
         }
         Decomp d = decompose(nact, pl.iw, ntiles(e, c), pl.grid, e.max_split);
@@ -209,7 +208,7 @@ __device__ __forceinline__ ClPlan plan_cluster(const Ens &e, uint32_t c, uint32_
         pl.tpc = d.tiles_per_chunk;
         pl.n_it = d.n_it;
         pl.rows = d.rows;
-        pl.lanes = finish_lanes(d.S);
+        pl.lanes = finish_lanes(d.S, nact);
     }
     return pl;
 }
@@ -670,8 +669,10 @@ __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const u
         const uint32_t c = single ? 0u : find_cluster(e.items_pre, e.K, item);
         const ClPlan pl = single ? pl0 : e.plan[c];
         const uint32_t li = (uint32_t)(item - (single ? 0ull : e.items_pre[c]));
-        const uint32_t it = li / pl.S;
-        const uint32_t sc = li - it * pl.S;
+        // j-chunk major: items taken at about the same time share their j
+        // tiles (one chunk per XCD L2 rather than S of them)
+        const uint32_t sc = li / pl.n_it;
+        const uint32_t it = li - sc * pl.n_it;
         const uint32_t nact = e.cs[c].nact;
         const uint32_t jt_begin = e.off[c] / JT;
         const uint32_t jt_end = jt_begin + ntiles(e, c);
@@ -1281,12 +1282,12 @@ void drop_graph(h4h_ctx *ctx) {
 // (Re)sizes the j-split partials for the current grid and updates the
 // launch fields of the cluster table.
 int configure_launch(h4h_ctx *ctx) {
-    const uint32_t target = ctx->grid * QUEUE_FACTOR;
     uint32_t g = ctx->grid;
     if (ctx->K == 1)
-        g = std::min(g, max_items(ctx->cnt[0], target, ctx->max_split, ctx->iw_fill));
+        g = std::min(g, max_items(ctx->cnt[0], ctx->grid, ctx->max_split, ctx->iw_fill));
     ctx->e.grid = g;
-    ctx->e.target = target;
+    // queue target of the plans: an upper bound of their work items
+    const uint32_t target = ctx->grid * QUEUE_FACTOR;
     ctx->e.max_split = ctx->max_split;
     ctx->e.iw_fill = ctx->iw_fill;
     drop_graph(ctx);

@@ -15,38 +15,39 @@ using namespace h4;
 
 static int failures = 0;
 
-static void check_single(uint32_t n, uint32_t target, uint32_t max_split) {
+static void check_single(uint32_t n, uint32_t grid, uint32_t max_split) {
     const uint32_t n_jt = (n + JT - 1) / JT;
     const uint32_t widths[4] = {4, 16, 64, 256};
-    const unsigned long long bound = partial_bound(n, target);
+    const unsigned long long bound = partial_bound(n, grid * QUEUE_FACTOR);
     for (uint32_t nact = 1; nact <= n; ++nact) {
         for (uint32_t iw : widths) {
-            const Decomp d = decompose(nact, iw, n_jt, target, max_split);
+            const Decomp d = decompose(nact, iw, n_jt, grid, max_split);
             const bool covers = d.S >= 1 && (unsigned long long)d.S * d.tiles_per_chunk >= n_jt &&
                                 (unsigned long long)(d.S - 1) * d.tiles_per_chunk < n_jt;
             const unsigned long long part = d.S > 1 ? 6ull * d.S * d.rows : 0ull;
             if (!covers || part > bound || d.n_items != d.n_it * d.S ||
                 d.rows != d.n_it * iw || (unsigned long long)d.n_it * iw < nact) {
                 if (failures++ < 10)
-                    std::printf("FAIL n=%u target=%u max_split=%u nact=%u iw=%u: S=%u tpc=%u "
+                    std::printf("FAIL n=%u grid=%u max_split=%u nact=%u iw=%u: S=%u tpc=%u "
                                 "rows=%u partials=%llu bound=%llu\n",
-                                n, target, max_split, nact, iw, d.S, d.tiles_per_chunk, d.rows,
+                                n, grid, max_split, nact, iw, d.S, d.tiles_per_chunk, d.rows,
                                 part, bound);
             }
         }
     }
 }
 
-static void check_ensembles(uint32_t trials, uint32_t target) {
-    std::mt19937_64 g(20261202);
+static void check_ensembles(uint32_t trials, uint32_t grid) {
+    const uint32_t target = grid * QUEUE_FACTOR;
+    std::mt19937_64 rng(20261202);
     for (uint32_t t = 0; t < trials; ++t) {
-        const uint32_t K = 2 + g() % 80;
+        const uint32_t K = 2 + rng() % 80;
         std::vector<uint32_t> n(K), nact(K);
         unsigned long long n_total = 0, per_cluster = 0;
         double W = 0.0;
         for (uint32_t c = 0; c < K; ++c) {
-            n[c] = 1 + g() % 40000;
-            nact[c] = (g() % 4 == 0) ? 0 : 1 + g() % n[c];
+            n[c] = 1 + rng() % 40000;
+            nact[c] = (rng() % 4 == 0) ? 0 : 1 + rng() % n[c];
             n_total += n[c];
             per_cluster += partial_bound(n[c], target);
             W += (double)nact[c] * (double)n[c];
@@ -56,7 +57,7 @@ static void check_ensembles(uint32_t trials, uint32_t target) {
             if (!nact[c]) continue;
             // the share rule of the device plan (plan_cluster)
             const double share = W > 0.0 ? (double)nact[c] * (double)n[c] / W : 0.0;
-            uint32_t gc = (uint32_t)(share * target);
+            uint32_t gc = (uint32_t)(share * grid);
             if (gc < 1) gc = 1;
             const Decomp d = decompose(nact[c], pick_iw(nact[c], DEFAULT_IW_FILL),
                                        (n[c] + JT - 1) / JT, gc, 1024);
@@ -74,16 +75,16 @@ static void check_ensembles(uint32_t trials, uint32_t target) {
 
 int main() {
     const uint32_t sizes[] = {1, 2, 3, 255, 256, 257, 1000, 1024, 4097, 65536};
-    const uint32_t targets[] = {1, 7, 256, 2048, 32768, 65536};
+    const uint32_t grids[] = {1, 7, 256, 2048, 4096};
     const uint32_t splits[] = {1, 16, 1024, 1u << 30};
     for (uint32_t n : sizes)
-        for (uint32_t tg : targets)
-            for (uint32_t ms : splits) check_single(n, tg, ms);
-    check_single(262144, 32768, 1024);
+        for (uint32_t gr : grids)
+            for (uint32_t ms : splits) check_single(n, gr, ms);
     check_single(262144, 2048, 1024);
-    check_single(300000, 32768, 1024);
-    check_ensembles(2000, 65536);
-    check_ensembles(2000, 2048);
+    check_single(262144, 128, 1024);
+    check_single(300000, 2048, 1024);
+    check_ensembles(2000, 4096);
+    check_ensembles(2000, 256);
     if (failures) {
         std::printf("%d failures\n", failures);
         return 1;
