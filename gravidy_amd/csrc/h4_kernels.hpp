@@ -85,7 +85,8 @@ struct RunCtl {
     unsigned long long max_steps;  // stop after this many steps
     uint32_t ndirty;               // blocks with active particles this step
     uint32_t ndone;                // clusters that have stopped
-    uint32_t pad[2];
+    uint32_t qdone;                // update workgroups done with the work queue
+    uint32_t pad;
     unsigned long long items;      // update work items this step
     unsigned long long fin_units;  // finish workgroup units this step
     unsigned long long qnext;      // update work queue: items claimed beyond the

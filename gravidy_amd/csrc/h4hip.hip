@@ -661,8 +661,9 @@ __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const u
     const ClPlan pl0 = single ? plan_single(e, 0u, total) : ClPlan{};
     if (!single) total = e.ctl->items;
     // The first gridDim.x items go one to each workgroup; a workgroup that
-    // has finished one takes the next unclaimed item from the queue counter
-    // (zeroed again by the finish pass), so faster CUs and XCDs take more.
+    // has finished one takes the next unclaimed item from the queue counter,
+    // so faster CUs and XCDs take more.  The last workgroup to leave the
+    // queue zeroes it for the next launch.
     __shared__ unsigned long long claimed;
     unsigned long long item = blockIdx.x;
     while (item < total) {
@@ -697,6 +698,13 @@ __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const u
         if (threadIdx.x == 0) claimed = gridDim.x + atomicAdd(&e.ctl->qnext, 1ull);
         __syncthreads();
         item = claimed;
+    }
+    // Every workgroup claims until it draws an index >= total and then
+    // counts itself out; when all have, no claim of this launch remains.
+    if (total > gridDim.x && threadIdx.x == 0 &&
+        atomicAdd(&e.ctl->qdone, 1u) == gridDim.x - 1) {
+        atomicExch(&e.ctl->qnext, 0ull);
+        atomicExch(&e.ctl->qdone, 0u);
     }
 }
 
@@ -821,7 +829,6 @@ __global__ __launch_bounds__(BS) void k_finish(Soa s, Ens e, const uint32_t *mov
                                               const double *partial, uint32_t mode, float eta,
                                               double dt_min, double dt_max) {
     const unsigned long long total = e.ctl->fin_units;
-    if (blockIdx.x == 0 && threadIdx.x == 0) e.ctl->qnext = 0ull;  // the update is over
     for (unsigned long long u = blockIdx.x; u < total; u += gridDim.x) {
         const uint32_t c = find_cluster(e.fin_pre, e.K, u);
         const ClPlan pl = e.plan[c];
