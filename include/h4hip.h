@@ -90,7 +90,9 @@ int h4h_destroy(h4h_ctx *ctx);
 int h4h_set_precision(h4h_ctx *ctx, int precision);
 
 /* Tuning knobs of the FP64/mixed update kernel: persistent workgroups
- * (0 = default) and the largest j-split factor (0 = default). */
+ * (0 = default 2048; a step's work items beyond one per workgroup, up to
+ * 16 per workgroup, are handed out by the kernel's work queue) and the
+ * largest j-split factor (0 = default 1024). */
 int h4h_set_launch(h4h_ctx *ctx, uint32_t workgroups, uint32_t max_jsplit);
 
 /* Copies the state to the device.  r4 and v4 are required; f6, t, dt may
