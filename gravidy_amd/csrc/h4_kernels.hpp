@@ -139,7 +139,9 @@ __host__ __device__ inline double dkey_inv(unsigned long long k) {
 // workgroup (taken from the update kernel's work queue, which evens out
 // faster and slower CUs and XCDs) as long as a chunk keeps CHUNK_TILES
 // tiles (more chunks would mostly add partial sums to write and read).  A
-// chunk holds at least 64/IW tiles so one item is >= 16K pairs.
+// chunk of a narrow group (IW 4 or 16) holds at least 4 tiles, 4K or 16K
+// pairs: shorter items cost more in partial sums than they gain (IW 16),
+// longer ones leave small steps waiting on one item's tile copies (IW 4).
 constexpr uint32_t QUEUE_FACTOR = 16;
 constexpr uint32_t CHUNK_TILES = 16;
 
@@ -156,7 +158,7 @@ __host__ __device__ inline Decomp decompose(uint32_t nact, uint32_t iw,
                                             uint32_t max_split) {
     Decomp d;
     d.n_it = (nact + iw - 1) / iw;
-    uint32_t min_tiles = iw >= 64 ? 1u : 64u / iw;
+    uint32_t min_tiles = iw >= 64 ? 1u : 4u;
     uint32_t S = 1;
     const uint32_t target = grid * QUEUE_FACTOR;
     if (d.n_it > 0 && d.n_it < target) {
