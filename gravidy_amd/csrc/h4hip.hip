@@ -399,12 +399,14 @@ __global__ __launch_bounds__(BS) void k_save_old(Soa s, Ens e, const uint32_t *m
 // Pairwise acceleration + jerk
 // ---------------------------------------------------------------------------
 
-// Loads JT consecutive predictor records starting at j0 into LDS; each of
-// the BS (= JT) threads moves one record (two coalesced double4 loads).
-__device__ __forceinline__ void load_jtile(const Soa &s, uint32_t j0,
-                                           double4 *sp, double4 *sv) {
-    sp[threadIdx.x] = s.pp[j0 + threadIdx.x];
-    sv[threadIdx.x] = s.pv[j0 + threadIdx.x];
+// Loads JT consecutive predictor records starting at j0 into an LDS tile;
+// each of the BS (= JT) threads moves one record (two coalesced double4
+// loads).  ES = 1: pp records at [0, JT), pv records at [JT, 2 JT);
+// ES = 2: record t at [2t] (pp) and [2t + 1] (pv).
+template <uint32_t ES = 1>
+__device__ __forceinline__ void load_jtile(const Soa &s, uint32_t j0, double4 *tile) {
+    tile[ES * threadIdx.x] = s.pp[j0 + threadIdx.x];
+    tile[ES == 2 ? 2 * threadIdx.x + 1 : JT + threadIdx.x] = s.pv[j0 + threadIdx.x];
 }
 
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -450,11 +452,13 @@ __device__ __forceinline__ float mass_f32(const double4 &b) {
 
 // Inner loop of one lane over one LDS tile: the lane's j sub-group js
 // takes tile entries js, js + G, ..., G = BS/IW (all JT entries when
-// IW = BS).  SELF: the tile may hold the lane's own i, whose term is
+// IW = BS).  Entry jj is sp[ES jj] (position, mass) and sv[ES jj]
+// (velocity): ES = 1 for a tile stored as two arrays, 2 for one stored as
+// interleaved record pairs.  SELF: the tile may hold the lane's own i, whose term is
 // skipped (mass 0, and r2 kept finite when e2 == 0).  MIXED: FP32 pair
 // terms summed in FP32 over runs of at most 32 pairs, each run then added
 // to the FP64 accumulators.
-template <int IW, bool SELF, bool MIXED>
+template <int IW, bool SELF, bool MIXED, uint32_t ES>
 __device__ __forceinline__ void tile_pairs(const double pi[6], uint32_t i,
                                            uint32_t j0, uint32_t js,
                                            const double4 *sp, const double4 *sv,
@@ -465,8 +469,8 @@ __device__ __forceinline__ void tile_pairs(const double pi[6], uint32_t i,
 #pragma unroll 4
         for (uint32_t q = 0; q < NQ; ++q) {
             uint32_t jj = js + q * G;
-            double4 a = sp[jj];
-            double4 b = sv[jj];
+            double4 a = sp[ES * jj];
+            double4 b = sv[ES * jj];
             if (SELF) {
                 bool me = (j0 + jj) == i;
                 double mj = me ? 0.0 : a.w;
@@ -484,8 +488,8 @@ __device__ __forceinline__ void tile_pairs(const double pi[6], uint32_t i,
 #pragma unroll H4_MIXED_UNROLL
             for (uint32_t q = q0; q < q0 + FOLD; ++q) {
                 uint32_t jj = js + q * G;
-                double4 a = sp[jj];
-                double4 b = sv[jj];
+                double4 a = sp[ES * jj];
+                double4 b = sv[ES * jj];
                 if (SELF) {
                     bool me = (j0 + jj) == i;
                     float mj = me ? 0.f : mass_f32(b);
@@ -540,8 +544,8 @@ __device__ __forceinline__ void reduce_subgroups(double fo[6], double *red) {
 // Writes f[i] directly.  Workgroups stride over the plan's i tiles.
 __global__ __launch_bounds__(BS) void k_update_exact(Soa s, Ens e, const uint32_t *move,
                                                     double e2) {
-    __shared__ double4 sp[JT];
-    __shared__ double4 sv[JT];
+    __shared__ double4 tile[2 * JT];
+    const double4 *sp = tile, *sv = tile + JT;
     const bool single = e.K == 1;
     unsigned long long total = 0;
     if (single)
@@ -562,7 +566,7 @@ __global__ __launch_bounds__(BS) void k_update_exact(Soa s, Ens e, const uint32_
         double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (uint32_t j0 = j_begin; j0 < j_end; j0 += JT) {
             __syncthreads();
-            load_jtile(s, j0, sp, sv);
+            load_jtile(s, j0, tile);
             __syncthreads();
             uint32_t jn = min((uint32_t)JT, j_end - j0);
             for (uint32_t jj = 0; jj < jn; ++jj) {
@@ -596,13 +600,19 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
     double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     uint32_t t0 = jt_begin + sc * pl.tpc;
     uint32_t t1 = min(jt_end, t0 + pl.tpc);
+    // FP64 wide groups keep their tile as interleaved (position, velocity)
+    // record pairs: every LDS read of the pair loop then fits the
+    // instructions' offset field, with no extra address arithmetic (0.5 %
+    // on the full step; the mixed kernel runs as fast with two arrays).
+    constexpr uint32_t ES = (IW >= 64 && !MIXED) ? 2u : 1u;
     auto tile = [&](const double4 *buf, uint32_t tt) {
         const uint32_t j0 = tt * JT;
         const bool self = (i - j0) < (uint32_t)JT;
+        const double4 *sv = ES == 2 ? buf + 1 : buf + JT;
         if (__ballot(self))
-            tile_pairs<IW, true, MIXED>(pi, i, j0, js, buf, buf + JT, e2, fo);
+            tile_pairs<IW, true, MIXED, ES>(pi, i, j0, js, buf, sv, e2, fo);
         else
-            tile_pairs<IW, false, MIXED>(pi, i, j0, js, buf, buf + JT, e2, fo);
+            tile_pairs<IW, false, MIXED, ES>(pi, i, j0, js, buf, sv, e2, fo);
     };
     // Narrow lane groups (few pairs per lane and tile): double-buffered
     // tiles, the copy of tile tt + 1 in flight while tile tt is computed, one
@@ -629,7 +639,7 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
         // a small share of the time and one buffer keeps registers free.
         for (uint32_t tt = t0; tt < t1; ++tt) {
             __syncthreads();
-            load_jtile(s, tt * JT, lA, lA + JT);
+            load_jtile<ES>(s, tt * JT, lA);
             __syncthreads();
             tile(lA, tt);
         }
