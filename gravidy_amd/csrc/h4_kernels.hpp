@@ -270,16 +270,6 @@ __device__ __forceinline__ void pair_exact(const double pi[6], double xj,
     fo[5] += (vz * mr3inv - (3 * rv) * rz * mr5inv);
 }
 
-// 1/sqrt(x) for x > 0: v_rsq_f64 estimate y, then one third-order step of
-// the binomial series (1 - e)^(-1/2) = 1 + e/2 + 3e^2/8 + ..., e = 1 - x y^2,
-// which leaves about one rounding of error (used by the energy sum).
-__device__ __forceinline__ double rsqrt_fp64(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    double e = fma(-x, y * y, 1.0);
-    double p = fma(e, 0.375, 0.5);
-    return fma(y * e, p, y);
-}
-
 // The FP64 pair loop works with s = sqrt(3) / r and q = s^2 = 3 / r^2:
 // the jerk coefficient 3 (r.v) / r^2 is then (r.v) q, with no multiply by
 // 3, and m s q = 3 sqrt(3) m / r^3, so every pair term it sums is

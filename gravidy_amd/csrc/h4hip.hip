@@ -1083,13 +1083,15 @@ __global__ void k_set_times_from_tmin(Ens e, double itime) {
 // ---------------------------------------------------------------------------
 // Energy: sum 1/2 m v^2 - sum_{i<j} m_i m_j / |r_ij| (unsoftened) of one
 // cluster, NbodyUtils::get_energy (NbodyUtils.cpp:427-529).  1/|r_ij| from
-// rsqrt_fp64 (relative error near one rounding, against the reference's
-// division by sqrt): a lane sums m_j / |r_ij| over one j tile in plain FP64
-// (at most 256 terms of one sign, so at most 256 roundings of the partial
-// sum: 3e-14 relative), multiplies by m_i and adds that into its
-// Neumaier-compensated sum; then per workgroup, then over workgroups in
-// index order (deterministic for a fixed grid).  Coincident particles give
-// -inf as in the reference.
+// v_rsq_f64 and one Newton step (relative error below 2^-47 and of one
+// sign, against the reference's division by sqrt): a lane sums
+// m_j / |r_ij| over one j tile in plain FP64 (at most 256 terms of one
+// sign, so at most 256 roundings of the partial sum: 3e-14 relative),
+// multiplies by m_i and adds that into its Neumaier-compensated sum; then
+// per workgroup, then over workgroups in index order (deterministic for a
+// fixed grid).  A coincident pair (r = 0) makes the Newton step NaN; such
+// a tile is summed again with 1/0 = inf, so coincident particles give -inf
+// as in the reference.
 // ---------------------------------------------------------------------------
 
 // Neumaier-compensated sum += x.  Once the sum is infinite (coincident
@@ -1162,16 +1164,24 @@ __global__ __launch_bounds__(BS) void k_energy(Soa s, uint32_t off, uint32_t n, 
             double xi = s.r[0][i], yi = s.r[1][i], zi = s.r[2][i], mi = s.m[i];
             uint32_t jn = min((uint32_t)JT, n - lj0);
             uint32_t jb = (jt == it) ? threadIdx.x + 1 : 0;
-            double tile = 0.0;
-            for (uint32_t jj = jb; jj < jn; ++jj) {
-                double4 q = sr[jj];
-                double rx = q.x - xi;
-                double ry = q.y - yi;
-                double rz = q.z - zi;
-                double r2 = __builtin_fma(rx, rx, __builtin_fma(ry, ry, rz * rz));
-                double rinv = r2 > 0.0 ? rsqrt_fp64(r2) : INFINITY;
-                tile = __builtin_fma(q.w, rinv, tile);
-            }
+            auto tile_sum = [&](bool guard) {
+                double sum = 0.0;
+                for (uint32_t jj = jb; jj < jn; ++jj) {
+                    double4 q = sr[jj];
+                    double rx = q.x - xi;
+                    double ry = q.y - yi;
+                    double rz = q.z - zi;
+                    double r2 = __builtin_fma(rx, rx, __builtin_fma(ry, ry, rz * rz));
+                    double y = __builtin_amdgcn_rsq(r2);
+                    double e = __builtin_fma(-r2, y * y, 1.0);
+                    double rinv = __builtin_fma(y * e, 0.5, y);
+                    if (guard && !(r2 > 0.0)) rinv = INFINITY;
+                    sum = __builtin_fma(q.w, rinv, sum);
+                }
+                return sum;
+            };
+            double tile = tile_sum(false);
+            if (isnan(tile)) tile = tile_sum(true);
             if (jb < jn) neumaier(ep, epc, -(mi * tile));
             if (jt == it) {
                 double vx = s.v[0][i], vy = s.v[1][i], vz = s.v[2][i];
