@@ -156,11 +156,20 @@ def run_leg(e2, n, seed, t_end, precision, cpu=False):
         t0 = time.perf_counter()
         run = h.integration(t_end)
         el = time.perf_counter() - t0
+        small = h.stats()["small_steps"]
     e = run["energy"]
     out = {"n": n, "seed": seed, "t_end": t_end, "precision": precision,
            "block_steps": int(len(run["step_time"])), "interactions": int(run["interactions"]),
            "seconds": el, "pairs_per_s": run["interactions"] / el,
-           "cum_energy_error": float(abs((e[-1] - e[0]) / e[0]))}
+           "cum_energy_error": float(abs((e[-1] - e[0]) / e[0])),
+           "one_workgroup_steps": int(small)}
+    if small:
+        # the same run with every block step on the multi-workgroup kernels
+        with Hermite4HIP(r, v, e2, precision=precision) as h:
+            h.set_small_path(0)
+            t0 = time.perf_counter()
+            h.integration(t_end)
+            out["seconds_multi_workgroup_only"] = time.perf_counter() - t0
     if cpu:
         from oracle import Oracle
         o = Oracle()

@@ -33,7 +33,8 @@ EXPORTS = (
     "h4h_next_integration_time", "h4h_step", "h4h_run", "h4h_get_energy",
     "h4h_set_timing", "h4h_get_stats", "h4h_synchronize", "h4h_create_ensemble",
     "h4h_ensemble_sizes", "h4h_ensemble_init", "h4h_ensemble_run", "h4h_ensemble_energy",
-    "h4h_set_tuning", "h4h_autotune", "h4h_set_graphs", "h4h_tune_launch",
+    "h4h_set_tuning", "h4h_autotune", "h4h_set_graphs", "h4h_set_small_path",
+    "h4h_tune_launch",
     "h4h_set_launch_choice",
 )
 
@@ -46,6 +47,7 @@ class H4HStats(ctypes.Structure):
         ("update_launches", ctypes.c_uint64),
         ("update_ms", ctypes.c_double),
         ("step_ms", ctypes.c_double),
+        ("small_steps", ctypes.c_uint64),
     ]
 
 
@@ -146,6 +148,7 @@ def load():
         "h4h_set_tuning": ([P, pu, pu], i32),
         "h4h_autotune": ([P, u32, pu, pu], i32),
         "h4h_set_graphs": ([P, i32], i32),
+        "h4h_set_small_path": ([P, u32], i32),
         "h4h_tune_launch": ([P, u32, ctypes.POINTER(H4HLaunchChoice)], i32),
         "h4h_set_launch_choice": ([P, ctypes.POINTER(H4HLaunchChoice)], i32),
     }

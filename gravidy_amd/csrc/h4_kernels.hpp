@@ -63,6 +63,7 @@ struct ClState {
     unsigned long long steps;         // block steps since the last setup
     unsigned long long interactions;  // sum of nact * n
     unsigned long long nsteps;        // sum of nact
+    unsigned long long small_steps;   // block steps made by the one-workgroup kernel
 };
 
 // Work plan of one step for one cluster (computed on the device after the

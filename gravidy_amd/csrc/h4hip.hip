@@ -319,6 +319,45 @@ enum : uint32_t {
 // The work plan is made by k_plan afterwards: a last-workgroup pattern
 // here would need a device-scope fence per workgroup, i.e. an L2
 // write-back on every workgroup of this multi-XCD part.
+// Hermite4CPU::predicted_pos_vel (Hermite4CPU.cpp:340-365) of one particle
+// (position r, velocity v, forces f, float mass mf) over dt = itime - t,
+// as the two predictor records.
+__device__ __forceinline__ void predict_vals(const double r[3], const double v[3],
+                                             const double f[6], double dt, float mf,
+                                             double4 &a, double4 &b) {
+#pragma clang fp contract(off)
+    double dt2 = (dt * dt);
+    double dt3 = (dt2 * dt);
+    double dt2c = dt2 / 2.0;
+    double dt3c = dt3 / 6.0;
+    double pr[3], pvel[3];
+    for (int k = 0; k < 3; ++k) {
+        pr[k] = (dt3c * f[3 + k]) + (dt2c * f[k]) + (dt * v[k]) + r[k];
+        pvel[k] = (dt2c * f[3 + k]) + (dt * f[k]) + v[k];
+    }
+    a = make_double4(pr[0], pr[1], pr[2], (double)mf);
+    b = make_double4(pvel[0], pvel[1], pvel[2], mass_bits(mf));
+}
+
+// Predictor of particle i (time ti) at itime, stored in s.pp / s.pv;
+// returns it too.
+__device__ __forceinline__ void predict_one(const Soa &s, uint32_t i, double ti, double itime,
+                                            double4 *pp = nullptr, double4 *pv = nullptr) {
+#pragma clang fp contract(off)
+    double f[6], r[3], v[3];
+    for (int k = 0; k < 6; ++k) f[k] = s.f[k][i];
+    for (int k = 0; k < 3; ++k) {
+        r[k] = s.r[k][i];
+        v[k] = s.v[k][i];
+    }
+    double4 a, b;
+    predict_vals(r, v, f, itime - ti, (float)s.m[i], a, b);
+    s.pp[i] = a;
+    s.pv[i] = b;
+    if (pp) *pp = a;
+    if (pv) *pv = b;
+}
+
 __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, double itime_arg,
                                               uint32_t *move) {
 #pragma clang fp contract(off)
@@ -333,27 +372,7 @@ __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, dou
     bool act = false;
     if (in) {
         double ti = s.t[i];
-        if (flags & PS_PREDICT) {
-            double f[6], r[3], v[3];
-            for (int k = 0; k < 6; ++k) f[k] = s.f[k][i];
-            for (int k = 0; k < 3; ++k) {
-                r[k] = s.r[k][i];
-                v[k] = s.v[k][i];
-            }
-            double dt = itime - ti;
-            double dt2 = (dt * dt);
-            double dt3 = (dt2 * dt);
-            double dt2c = dt2 / 2.0;
-            double dt3c = dt3 / 6.0;
-            double pr[3], pv[3];
-            for (int k = 0; k < 3; ++k) {
-                pr[k] = (dt3c * f[3 + k]) + (dt2c * f[k]) + (dt * v[k]) + r[k];
-                pv[k] = (dt2c * f[3 + k]) + (dt * f[k]) + v[k];
-            }
-            const float mf = (float)s.m[i];
-            s.pp[i] = make_double4(pr[0], pr[1], pr[2], (double)mf);
-            s.pv[i] = make_double4(pv[0], pv[1], pv[2], mass_bits(mf));
-        }
+        if (flags & PS_PREDICT) predict_one(s, i, ti, itime);
         if (flags & PS_SELECT) {
             double tmp_time = ti + s.dt[i];
             act = fabs(itime - tmp_time) < 2 * DBL_EPSILON;
@@ -771,16 +790,16 @@ __device__ __forceinline__ double normalize_dt(double new_dt, double old_dt,
 }
 
 // Hermite4CPU::correction_pos_vel (Hermite4CPU.cpp:370-411) with
-// NbodyUtils::get_timestep_normal (NbodyUtils.cpp:337-365) for particle i
-// at block time itime, given its new forces f[6] (old forces in s.old,
-// predictor in s.pp / s.pv).  Returns the new dt.
-__device__ __forceinline__ void correct_one(const Soa &s, uint32_t i, const double f[6],
-                                            double itime, float eta, double dt_min,
-                                            double dt_max) {
+// NbodyUtils::get_timestep_normal (NbodyUtils.cpp:337-365) of one particle
+// at block time itime: old forces o, new forces f, predictor pr (position,
+// velocity), step dt1, ETA (the reference's float, widened exactly).  Writes
+// the corrected position rn and velocity vn and a2, a3; returns the new step.
+__device__ __forceinline__ double correct_vals(const double o[6], const double f[6],
+                                               const double pr[6], double dt1, double itime,
+                                               double eta, double dt_min, double dt_max,
+                                               double rn[3], double vn[3], double a2[3],
+                                               double a3[3]) {
 #pragma clang fp contract(off)
-    double o[6], a2[3], a3[3];
-    for (int c = 0; c < 6; ++c) o[c] = s.old[c][i];
-    double dt1 = s.dt[i];
     double dt2 = dt1 * dt1;
     double dt3 = dt2 * dt1;
     double dt4 = dt2 * dt2;
@@ -789,16 +808,10 @@ __device__ __forceinline__ void correct_one(const Soa &s, uint32_t i, const doub
         a2[c] = (-6 * (o[c] - f[c]) - dt1 * (4 * o[3 + c] + 2 * f[3 + c])) / dt2;
     for (int c = 0; c < 3; ++c)
         a3[c] = (12 * (o[c] - f[c]) + 6 * dt1 * (o[3 + c] + f[3 + c])) / dt3;
-    double pr[6];
-    load_predictor(s, i, pr);
     for (int c = 0; c < 3; ++c) {
-        s.r[c][i] = pr[c] + (dt4 / 24) * a2[c] + (dt5 / 120) * a3[c];
-        s.v[c][i] = pr[3 + c] + (dt3 / 6) * a2[c] + (dt4 / 24) * a3[c];
-        s.a2[c][i] = a2[c];
-        s.a3[c][i] = a3[c];
+        rn[c] = pr[c] + (dt4 / 24) * a2[c] + (dt5 / 120) * a3[c];
+        vn[c] = pr[3 + c] + (dt3 / 6) * a2[c] + (dt4 / 24) * a3[c];
     }
-    s.t[i] = itime;
-
     double ax1_2 = a2[0] + dt1 * a3[0];
     double ay1_2 = a2[1] + dt1 * a3[1];
     double az1_2 = a2[2] + dt1 * a3[2];
@@ -810,7 +823,29 @@ __device__ __forceinline__ void correct_one(const Soa &s, uint32_t i, const doub
     double abs_a1_22 = abs_a1_2 * abs_a1_2;
     double normal_dt = sqrt(eta * ((abs_a1 * abs_a1_2 + abs_j12) /
                                    (abs_j1 * abs_a1_3 + abs_a1_22)));
-    s.dt[i] = normalize_dt(normal_dt, dt1, itime, dt_min, dt_max);
+    return normalize_dt(normal_dt, dt1, itime, dt_min, dt_max);
+}
+
+// The corrector of particle i at block time itime, given its new forces
+// f[6] (old forces in s.old, predictor in s.pp / s.pv), on the state in
+// HBM.  Returns the new dt.
+__device__ __forceinline__ double correct_one(const Soa &s, uint32_t i, const double f[6],
+                                              double itime, float eta, double dt_min,
+                                              double dt_max) {
+    double o[6], pr[6], rn[3], vn[3], a2[3], a3[3];
+    for (int c = 0; c < 6; ++c) o[c] = s.old[c][i];
+    load_predictor(s, i, pr);
+    const double dt_new = correct_vals(o, f, pr, s.dt[i], itime, eta, dt_min, dt_max, rn, vn,
+                                       a2, a3);
+    for (int c = 0; c < 3; ++c) {
+        s.r[c][i] = rn[c];
+        s.v[c][i] = vn[c];
+        s.a2[c][i] = a2[c];
+        s.a3[c][i] = a3[c];
+    }
+    s.t[i] = itime;
+    s.dt[i] = dt_new;
+    return dt_new;
 }
 
 // Standalone corrector of cluster 0's active list at itime (the ABI call
@@ -1004,6 +1039,7 @@ __global__ __launch_bounds__(BS) void k_run_setup(Ens e, double t_end, double t_
         cs.steps = 0;
         cs.interactions = 0;
         cs.nsteps = 0;
+        cs.small_steps = 0;
         cs.nact = 0;
         cs.done = (cs.itime < t_end) ? 0u : 1u;
         nd += cs.done;
@@ -1061,6 +1097,294 @@ __global__ __launch_bounds__(BS) void k_step_end(Ens e) {
             cs->done = 1;
             atomicAdd(&e.ctl->ndone, 1u);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Small systems: whole block steps in one workgroup
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t SB = 1024;         // threads of k_small_run, and the largest N it takes
+constexpr uint32_t SMALL_MAX_ACT = 256;  // largest active set k_small_run takes
+static_assert(SB % JT == 0 && SB / JT <= SB / 64, "block minima from the wave minima");
+
+// p (a global-memory pointer), as a value the compiler cannot hoist out of
+// a loop: stores through it inside the loop address SGPR base + VGPR offset
+// instead of holding one 64-bit VGPR address per array for the whole loop.
+// Typed as a global pointer so that they are global (not flat) stores: a
+// flat store also counts against the LDS wait of the next barrier.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *in_loop(T *p) {
+    asm volatile("" : "+s"(p));
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+// Number of threads of the workgroup with `flag` set, and (for those) the
+// count below them in thread order.  Ends with a barrier.
+__device__ __forceinline__ uint32_t small_count(bool flag, uint32_t *wcnt, uint32_t &below) {
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    const unsigned long long mask = __ballot(flag);
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    uint32_t total = 0;
+    below = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    for (uint32_t k = 0; k < SB / 64; ++k) {
+        const uint32_t c = wcnt[k];
+        below += k < w ? c : 0u;
+        total += c;
+    }
+    return total;
+}
+
+// Minimum of v over the workgroup (every thread gets it).  Ends after a
+// barrier; wmin is free again after the next one.
+__device__ __forceinline__ double small_min(double v, double *wmin) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    if (__lane_id() == 0) wmin[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double m = wmin[0];
+    for (uint32_t k = 1; k < SB / 64; ++k) m = fmin(m, wmin[k]);
+    return m;
+}
+
+// Block steps of a single system of n <= SB particles in one workgroup of SB
+// threads, up to max_run steps per launch.  Thread i owns particle i and
+// keeps its position, velocity, forces, time and step in registers for the
+// whole launch; the threads exchange the predictor records and the active
+// list through LDS, and the state goes to HBM as it changes (stores
+// that nothing in the launch reads back).  Each step is the device loop's
+// step: predictor of all particles at the block time T, the particles with
+// t + dt = T (Hermite4::find_particles_to_move), their old forces, the pair
+// sums, the corrector and new step, the next block time min(t + dt) and the
+// loop tail of k_step_end.  The active set of the next step is counted
+// before its predictor; when it is empty or larger than max_act the launch
+// ends and leaves that step to the multi-workgroup kernels.
+//
+// Pair sums: P = nact rounded up to a power of two i slots, G = SB / P lanes
+// per slot (lane g of slot p takes j = g, g + G, ...; P >= 64 puts one j per
+// wave, an LDS broadcast), reduced by shuffles inside a wave and in wave
+// order through LDS: a fixed order for a given nact.  The mixed path folds
+// its FP32 runs into FP64 every 32 pairs, as the main kernel does.  On exit
+// the predictor records of the last step and the per-block minima of t + dt
+// are stored for the kernels that follow.
+template <bool MIXED>
+__global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run,
+                                                  uint32_t max_act, double e2, double eta,
+                                                  double dt_min, double dt_max) {
+#pragma clang fp contract(off)
+    __shared__ double4 lpp[SB], lpv[SB];  // predictor records
+    __shared__ __align__(16) double red[6 * SB];  // pair sums per (group, slot); corrector output
+    __shared__ double stg[7 * SMALL_MAX_ACT];     // old forces and step per active slot
+    __shared__ uint32_t lact[SB];          // active list
+    __shared__ uint32_t wcnt[SB / 64];
+    __shared__ double wmin[SB / 64];
+
+    const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
+    ClState *cs = &e.cs[0];
+    const uint32_t n = e.cnt[0];
+    const bool in = tid < n;
+    if (max_act > SMALL_MAX_ACT) max_act = SMALL_MAX_ACT;
+    if (cs->done) return;
+    const double t_end = e.ctl->t_end, t_stop = e.ctl->t_stop;
+    const unsigned long long max_steps = e.ctl->max_steps;
+    unsigned long long steps = cs->steps;
+    double T = cs->atime;
+
+    double r[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+    double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double ti = 0.0, dti = 0.0;
+    float mf = 0.0f;
+    if (in) {
+        for (int c = 0; c < 3; ++c) {
+            r[c] = s.r[c][tid];
+            v[c] = s.v[c][tid];
+        }
+        for (int c = 0; c < 6; ++c) f[c] = s.f[c][tid];
+        ti = s.t[tid];
+        dti = s.dt[tid];
+        mf = (float)s.m[tid];
+    }
+    bool act = in && fabs(T - (ti + dti)) < 2 * DBL_EPSILON;
+    uint32_t k_act;  // this particle's place in the active list
+    uint32_t nact = small_count(act, wcnt, k_act);
+
+    uint32_t it = 0;
+    for (; it < max_run && nact >= 1 && nact <= max_act; ++it) {
+        // Predictor of every particle at T; active list.
+        if (in) {
+            double4 a, b;
+            predict_vals(r, v, f, T - ti, mf, a, b);
+            lpp[tid] = a;
+            lpv[tid] = b;
+        }
+        if (act) {
+            lact[k_act] = tid;
+            for (int c = 0; c < 6; ++c) stg[k_act * 7 + c] = f[c];
+            stg[k_act * 7 + 6] = dti;
+        }
+        __syncthreads();
+
+        // Pair sums of the active particles.
+        const uint32_t lp = nact > 1 ? 32u - (uint32_t)__clz(nact - 1) : 0u;
+        const uint32_t P = 1u << lp, G = SB >> lp;
+        const uint32_t slot = tid & (P - 1), g = tid >> lp;
+        const uint32_t i = lact[slot < nact ? slot : nact - 1];
+        double pi[6];
+        {
+            const double4 a = lpp[i], b = lpv[i];
+            pi[0] = a.x;
+            pi[1] = a.y;
+            pi[2] = a.z;
+            pi[3] = b.x;
+            pi[4] = b.y;
+            pi[5] = b.z;
+        }
+        double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if constexpr (!MIXED) {
+            for (uint32_t j = g; j < n; j += G) {
+                const double4 a = lpp[j], b = lpv[j];
+                const bool me = j == i;
+                pair_fp64(pi, me ? pi[0] + 1.0 : a.x, a.y, a.z, b.x, b.y, b.z, me ? 0.0 : a.w, e2,
+                          fo);
+            }
+        } else {
+            const float e2f = (float)e2;
+            for (uint32_t j0 = g; j0 < n; j0 += 32 * G) {
+                const uint32_t j1 = min(n, j0 + 32 * G);
+                float t6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                for (uint32_t j = j0; j < j1; j += G) {
+                    const double4 a = lpp[j], b = lpv[j];
+                    const bool me = j == i;
+                    pair_mixed(pi, me ? pi[0] + 1.0 : a.x, a.y, a.z, b.x, b.y, b.z,
+                               me ? 0.f : (float)a.w, e2f, t6);
+                }
+                for (int c = 0; c < 6; ++c) fo[c] += (double)t6[c];
+            }
+        }
+        if (P < 64)
+            for (uint32_t o = P; o < 64; o <<= 1)
+                for (int c = 0; c < 6; ++c) fo[c] += __shfl_xor(fo[c], (int)o);
+        // P <= 64: wave w holds the sums of slots 0..P-1 in lanes 0..P-1
+        // (group w); P > 64: every lane holds one slot, group g.
+        const uint32_t ngrp = P <= 64 ? SB / 64 : G;
+        if (P > 64 || lane < P) {
+            const uint32_t grp = P <= 64 ? w : g;
+            for (int c = 0; c < 6; ++c) red[(grp * P + slot) * 6 + c] = fo[c];
+        }
+        __syncthreads();
+
+        // Corrector of active slot tid (threads 0 .. nact-1, the first waves:
+        // each wave that holds an active slot runs the whole corrector).
+        double fn[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double rn[3], vn[3], dtn = 0.0;
+        if (tid < nact) {
+            // groups in order; ngrp <= SB / 64 (unrolled: the loads issue together)
+#pragma unroll
+            for (uint32_t q = 0; q < SB / 64; ++q) {
+                const bool use = q < ngrp;
+                const double2 *rp =
+                    reinterpret_cast<const double2 *>(red + ((use ? q : 0u) * P + tid) * 6);
+                const double2 x0 = rp[0], x1 = rp[1], x2 = rp[2];
+                fn[0] = use ? fn[0] + x0.x : fn[0];
+                fn[1] = use ? fn[1] + x0.y : fn[1];
+                fn[2] = use ? fn[2] + x1.x : fn[2];
+                fn[3] = use ? fn[3] + x1.y : fn[3];
+                fn[4] = use ? fn[4] + x2.x : fn[4];
+                fn[5] = use ? fn[5] + x2.y : fn[5];
+            }
+            if (!MIXED)
+                for (int c = 0; c < 6; ++c) fn[c] *= PAIR_SCALE;
+            const uint32_t ia = lact[tid];
+            double o[6], pr[6], a2[3], a3[3];
+            for (int c = 0; c < 6; ++c) o[c] = stg[tid * 7 + c];
+            {
+                const double4 a = lpp[ia], b = lpv[ia];
+                pr[0] = a.x;
+                pr[1] = a.y;
+                pr[2] = a.z;
+                pr[3] = b.x;
+                pr[4] = b.y;
+                pr[5] = b.z;
+            }
+            dtn = correct_vals(o, fn, pr, stg[tid * 7 + 6], T, eta, dt_min, dt_max, rn, vn, a2,
+                               a3);
+            for (int c = 0; c < 6; ++c) {
+                in_loop(s.old[c])[ia] = o[c];
+                in_loop(s.f[c])[ia] = fn[c];
+            }
+            for (int c = 0; c < 3; ++c) {
+                in_loop(s.r[c])[ia] = rn[c];
+                in_loop(s.v[c])[ia] = vn[c];
+                in_loop(s.a2[c])[ia] = a2[c];
+                in_loop(s.a3[c])[ia] = a3[c];
+            }
+            in_loop(s.t)[ia] = T;
+            in_loop(s.dt)[ia] = dtn;
+        }
+        __syncthreads();  // red is read: the corrected state goes there for the owners
+        if (tid < nact) {
+            double *out = red + tid * 13;
+            for (int c = 0; c < 6; ++c) out[c] = fn[c];
+            for (int c = 0; c < 3; ++c) {
+                out[6 + c] = rn[c];
+                out[9 + c] = vn[c];
+            }
+            out[12] = dtn;
+        }
+        __syncthreads();
+        if (act) {
+            const double *in_ = red + k_act * 13;
+            for (int c = 0; c < 6; ++c) f[c] = in_[c];
+            for (int c = 0; c < 3; ++c) {
+                r[c] = in_[6 + c];
+                v[c] = in_[9 + c];
+            }
+            ti = T;
+            dti = in_[12];
+        }
+
+        // Next block time (Hermite4::next_integration_time) and loop tail.
+        const double mn = small_min(in ? ti + dti : (double)INFINITY, wmin);
+        steps += 1;
+        const bool stop = T >= t_stop || steps >= max_steps || !(T < t_end) || !(mn > T);
+        if (tid == 0) {
+            e.hist_t[(steps - 1) % e.hist_cap] = T;
+            e.hist_n[(steps - 1) % e.hist_cap] = nact;
+            cs->itime = T;
+            cs->atime = mn;
+            cs->steps = steps;
+            cs->interactions += (unsigned long long)nact * n;
+            cs->nsteps += nact;
+            cs->small_steps += 1;
+            if (stop) {
+                cs->done = 1;
+                atomicAdd(&e.ctl->ndone, 1u);
+            }
+        }
+        T = mn;
+        if (stop) {
+            ++it;
+            break;
+        }
+        act = in && fabs(T - (ti + dti)) < 2 * DBL_EPSILON;
+        nact = small_count(act, wcnt, k_act);
+    }
+
+    // Predictor records of the last step made; per-block minima of t + dt.
+    if (it > 0 && in) {
+        s.pp[tid] = lpp[tid];
+        s.pv[tid] = lpv[tid];
+    }
+    __syncthreads();  // wmin is rewritten below
+    double vb = in ? ti + dti : (double)INFINITY;
+    for (int o = 32; o > 0; o >>= 1) vb = fmin(vb, __shfl_xor(vb, o));
+    if (lane == 0) wmin[w] = vb;
+    __syncthreads();
+    constexpr uint32_t WPB = JT / 64;  // waves per block
+    if (tid < e.nblk) {
+        double m = wmin[tid * WPB];
+        for (uint32_t k = 1; k < WPB; ++k) m = fmin(m, wmin[tid * WPB + k]);
+        e.bmin[tid] = m;
     }
 }
 
@@ -1208,8 +1532,13 @@ __global__ __launch_bounds__(BS) void k_energy(Soa s, uint32_t off, uint32_t n, 
 // ---------------------------------------------------------------------------
 
 namespace {
-constexpr uint32_t HIST_CAP = 64;   // per-step history ring of cluster 0 (>= RUN_CHUNK)
-constexpr uint32_t RUN_CHUNK = 32;  // block steps per captured graph
+constexpr uint32_t RUN_CHUNK = 32;    // block steps per captured graph
+constexpr uint32_t SMALL_RUN = 255;   // k_small_run steps per launch
+constexpr uint32_t SMALL_SLOTS = 8;   // (k_small_run + block step) pairs per captured graph
+constexpr uint32_t SMALL_ACT = 64;    // default largest active set k_small_run takes
+constexpr uint32_t HIST_CAP = 2048;   // per-step history ring of cluster 0
+static_assert(HIST_CAP >= RUN_CHUNK && HIST_CAP >= SMALL_SLOTS * (SMALL_RUN + 1),
+              "history ring shorter than the steps of one graph");
 constexpr int N_SOA = 35;           // doubles per particle in `state`
 }  // namespace
 
@@ -1230,6 +1559,8 @@ struct h4h_ctx {
     bool uploaded = false;
     bool timing = false;
     bool bm_valid = false;   // per-block minima match t + dt
+    bool small_ok = false;   // the device can launch k_small_run
+    uint32_t small_act = SMALL_ACT;  // k_small_run takes active sets up to this (0: off)
     uint32_t nact_host = 0;  // size of cluster 0's active list (single-system calls)
 
     hipStream_t stream = nullptr;
@@ -1375,6 +1706,34 @@ int launch_plan(h4h_ctx *ctx) {
     return H4H_OK;
 }
 
+// Whether the device can launch k_small_run (SB threads, 131 KiB of LDS).
+bool small_fits(int device) {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        return false;
+    for (const void *k : {reinterpret_cast<const void *>(&k_small_run<false>),
+                          reinterpret_cast<const void *>(&k_small_run<true>)}) {
+        hipFuncAttributes fa{};
+        if (hipFuncGetAttributes(&fa, k) != hipSuccess) return false;
+        if (fa.sharedSizeBytes > (size_t)lds || fa.maxThreadsPerBlock < (int)SB) return false;
+    }
+    return true;
+}
+
+// Device loops of this context start each step slot with k_small_run.
+bool small_path(const h4h_ctx *ctx) {
+    return ctx->small_ok && ctx->small_act > 0 && ctx->K == 1 && ctx->cnt[0] <= SB &&
+           ctx->precision != H4H_FP64_EXACT;
+}
+
+int launch_small(h4h_ctx *ctx, uint32_t max_run) {
+    auto *kern = ctx->precision == H4H_MIXED ? k_small_run<true> : k_small_run<false>;
+    hipLaunchKernelGGL(kern, dim3(1), dim3(SB), 0, ctx->stream, ctx->s, ctx->e, max_run,
+                       ctx->small_act, ctx->e2, (double)ctx->eta, ctx->dt_min, ctx->dt_max);
+    H4H_LAUNCHED();
+    return H4H_OK;
+}
+
 int ensure_blockmin(h4h_ctx *ctx) {
     if (ctx->bm_valid) return H4H_OK;
     hipLaunchKernelGGL(k_blockmin_all, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
@@ -1406,13 +1765,27 @@ int enqueue_step(h4h_ctx *ctx, bool events) {
     return H4H_OK;
 }
 
-// Captures RUN_CHUNK steps into one graph (once per configuration).  The
-// graph holds kernel nodes only.
+// One chunk of the device loop: RUN_CHUNK block steps, or for a small
+// system SMALL_SLOTS slots of k_small_run (up to SMALL_RUN steps with small
+// active sets) followed by one block step of the multi-workgroup kernels
+// (the step k_small_run left, if any).  At most HIST_CAP steps.
+int enqueue_chunk(h4h_ctx *ctx) {
+    int rc = H4H_OK;
+    if (small_path(ctx)) {
+        for (uint32_t k = 0; k < SMALL_SLOTS && rc == H4H_OK; ++k)
+            if ((rc = launch_small(ctx, SMALL_RUN)) == H4H_OK) rc = enqueue_step(ctx, false);
+    } else {
+        for (uint32_t k = 0; k < RUN_CHUNK && rc == H4H_OK; ++k) rc = enqueue_step(ctx, false);
+    }
+    return rc;
+}
+
+// Captures one chunk into a graph (once per configuration).  The graph
+// holds kernel nodes only.
 int ensure_graph(h4h_ctx *ctx) {
     if (ctx->graph_exec) return H4H_OK;
     H4H_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = H4H_OK;
-    for (uint32_t k = 0; k < RUN_CHUNK && rc == H4H_OK; ++k) rc = enqueue_step(ctx, false);
+    int rc = enqueue_chunk(ctx);
     hipGraph_t g = nullptr;
     hipError_t err = hipStreamEndCapture(ctx->stream, &g);
     if (rc) {
@@ -1631,6 +2004,7 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
     e.tune = ctx->d_tune;
     int rc = configure_launch(ctx);
     if (rc) return cleanup(rc);
+    ctx->small_ok = small_fits(ctx->device);
 
     double *b = ctx->state;
     Soa &s = ctx->s;
@@ -1672,7 +2046,7 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
 
 extern "C" {
 
-int h4h_abi_version(void) { return 105; }
+int h4h_abi_version(void) { return 106; }
 
 const char *h4h_last_error(void) { return g_err.c_str(); }
 
@@ -2002,6 +2376,8 @@ int h4h_step(h4h_ctx *ctx, double itime, uint32_t *nact, double *atime) {
     hipLaunchKernelGGL(k_run_setup, dim3(1), dim3(BS), 0, ctx->stream, ctx->e, (double)INFINITY,
                        -INFINITY, 1ull, 1u, -INFINITY, itime);
     H4H_LAUNCHED();
+    // The step the device loop would make: k_small_run when it takes it.
+    if (small_path(ctx) && (rc = launch_small(ctx, 1))) return rc;
     if ((rc = enqueue_step(ctx, ctx->timing))) return rc;
     if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     if ((rc = read_state(ctx))) return rc;
@@ -2019,6 +2395,7 @@ int h4h_step(h4h_ctx *ctx, double itime, uint32_t *nact, double *atime) {
     ctx->stats.block_steps += 1;
     ctx->stats.interactions += cs.interactions;
     ctx->stats.nsteps += k;
+    ctx->stats.small_steps += cs.small_steps;
     if (nact) *nact = k;
     if (atime) *atime = cs.atime;
     return H4H_OK;
@@ -2035,9 +2412,8 @@ static int run_chunks(h4h_ctx *ctx, double *step_time, uint32_t *step_nact, uint
     for (;;) {
         if (ctx->graphs) {
             H4H_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
-        } else {
-            for (uint32_t k = 0; k < RUN_CHUNK; ++k)
-                if ((rc = enqueue_step(ctx, false))) return rc;
+        } else if ((rc = enqueue_chunk(ctx))) {
+            return rc;
         }
         H4H_HIP(hipMemcpyAsync(ctx->ctl_host, ctx->d_ctl, sizeof(RunCtl), hipMemcpyDeviceToHost,
                                ctx->stream));
@@ -2106,6 +2482,7 @@ int h4h_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_steps, h4h_r
     ctx->stats.block_steps += d.steps;
     ctx->stats.interactions += d.interactions;
     ctx->stats.nsteps += d.nsteps;
+    ctx->stats.small_steps += d.small_steps;
     return H4H_OK;
 }
 
@@ -2357,6 +2734,14 @@ int h4h_set_graphs(h4h_ctx *ctx, int enable) {
     int rc = check_ctx(ctx, false);
     if (rc) return rc;
     ctx->graphs = enable != 0;
+    drop_graph(ctx);
+    return H4H_OK;
+}
+
+int h4h_set_small_path(h4h_ctx *ctx, uint32_t max_active) {
+    int rc = check_ctx(ctx, false);
+    if (rc) return rc;
+    ctx->small_act = max_active;
     drop_graph(ctx);
     return H4H_OK;
 }

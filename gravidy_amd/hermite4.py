@@ -143,6 +143,12 @@ class Hermite4HIP:
         """Device loops replay captured graphs (default) or launch directly."""
         check("h4h_set_graphs", self.lib.h4h_set_graphs(self._ctx, 1 if enable else 0))
 
+    def set_small_path(self, max_active=64):
+        """Block steps with 1..max_active (at most 256) active particles of a
+        system of at most 1024 run in one workgroup (0: every step on the
+        multi-workgroup kernels)."""
+        check("h4h_set_small_path", self.lib.h4h_set_small_path(self._ctx, int(max_active)))
+
     def stats(self, reset=False):
         s = _lib.H4HStats()
         check("h4h_get_stats", self.lib.h4h_get_stats(self._ctx, ctypes.byref(s), int(reset)))

@@ -60,6 +60,7 @@ typedef struct h4h_stats {
     uint64_t update_launches;  /* timed update passes                   */
     double   update_ms;        /* HIP-event time of the timed passes    */
     double   step_ms;          /* HIP-event time of timed h4h_step calls */
+    uint64_t small_steps;      /* block steps made in one workgroup     */
 } h4h_stats;
 
 /* Version of this ABI (major*100 + minor). */
@@ -290,6 +291,18 @@ int h4h_tune_launch(h4h_ctx *ctx, uint32_t steps, h4h_launch_choice *choice);
 /* Installs a launch choice (e.g. one h4h_tune_launch returned in an earlier
  * run); ms_per_step is ignored.  No counterpart in the reference. */
 int h4h_set_launch_choice(h4h_ctx *ctx, const h4h_launch_choice *choice);
+
+/* Small systems (one system of at most 1024 particles, FP64 or mixed
+ * precision): h4h_step and the device loop of h4h_run make every block step
+ * whose active set has 1 to max_active particles in one workgroup (each
+ * particle's state in one thread's registers, the predictor records and the
+ * active list in LDS), up to 255 steps per launch; larger steps go through
+ * the multi-workgroup kernels.  Default max_active 64; values above 256 act
+ * as 256; 0 turns the path off.  Same algorithm, same block-step schedule;
+ * the pair sums are added in another fixed order.  Ignored where the device
+ * cannot launch a 1024-thread workgroup with 131 KiB of LDS.  No
+ * counterpart in the reference (its GPU path launches kernels per step). */
+int h4h_set_small_path(h4h_ctx *ctx, uint32_t max_active);
 
 /* Enables HIP-event timing of update passes and steps (stats.update_ms,
  * stats.step_ms).  Off by default. */

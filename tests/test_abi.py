@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_error_text():
     lib = _lib.load()
-    assert lib.h4h_abi_version() == 105
+    assert lib.h4h_abi_version() == 106
     assert isinstance(lib.h4h_last_error(), bytes)
 
 

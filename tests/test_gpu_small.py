@@ -1,0 +1,147 @@
+"""The one-workgroup block step of small systems (N <= 1024, FP64 and
+mixed precision) against the oracle and against the multi-workgroup
+kernels.
+
+A step it makes must meet the same bars as the multi-workgroup step:
+predictor, old forces and times bit-identical to the oracle, forces within
+REL_TOL (FP64) or the mixed-precision statistics, and the same block-step
+schedule over whole runs.  The device loop, the loop launched without
+graphs and one h4h_step call per block step must agree bit for bit (each
+applies the same rule: a step with 1..max_active active particles is made
+in one workgroup).
+"""
+import numpy as np
+import pytest
+
+from gravidy_amd import Hermite4HIP, ic
+
+from test_gpu_parity import E2, bits_equal, block_state, check_forces, check_mixed
+
+pytestmark = pytest.mark.gpu
+
+# Block time at which only the particles with dt = 2^-12 are active
+# (about a tenth of a block_state).
+T_SMALL = 1.0 + 2.0 ** -12
+
+
+def small_state(n, seed):
+    r, v, f, t, dt = block_state(n, seed, T_SMALL)
+    dt[0] = 2.0 ** -12  # at least one active particle
+    t[0] = T_SMALL - dt[0]
+    return r, v, f, t, dt
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("n,seed,limit", [(1, 1, 256), (2, 2, 256), (3, 3, 256), (255, 4, 256),
+                                          (700, 5, 256), (1024, 6, 256), (1024, 6, "nact"),
+                                          (1024, 6, "nact-1"), (1024, 6, 0)])
+def test_small_step_matches_oracle(oracle, prec, n, seed, limit):
+    """One h4h_step from an injected state: the step is made in one
+    workgroup exactly when 1 <= nact <= max_active, and either way the
+    predictor, old forces and new times are bit-identical to the oracle's
+    and the forces are within the precision's bound."""
+    r, v, f, t, dt = small_state(n, seed)
+    mo = oracle.find_particles_to_move(T_SMALL, t, dt)
+    nact = int(mo.size)
+    assert 1 <= nact <= 256
+    max_active = {"nact": nact, "nact-1": nact - 1}.get(limit, limit)
+    with Hermite4HIP(r, v, E2, precision=prec) as h:
+        h.set_small_path(max_active)
+        h.upload(r, v, f, t, dt)
+        h.stats(reset=True)
+        k, atime = h.step(T_SMALL)
+        st = h.stats()
+        p, pm = h.download_predictor()
+        s = h.download()
+    assert k == nact
+    assert st["block_steps"] == 1
+    assert st["small_steps"] == (1 if nact <= max_active else 0)
+    po, pmo = oracle.predicted_pos_vel(T_SMALL, t, r, v, f)
+    assert bits_equal(p, po) and bits_equal(pm, pmo)
+    fo = oracle.update_acc_jrk(mo, po, pmo, E2, f)
+    label = f"small step n={n} nact={nact} max_active={max_active} {prec}"
+    if prec == "fp64":
+        check_forces(s["f"], fo, mo, po, pmo, E2, label)
+    elif not fo[mo].any():
+        assert not s["f"][mo].any()  # one particle: no pair
+    elif nact >= 64:
+        check_mixed(s["f"], fo, mo, label)
+    else:
+        # too few particles for the statistics: per-particle bounds only
+        ra = np.linalg.norm(s["f"][mo, :3] - fo[mo, :3], axis=1) / np.linalg.norm(fo[mo, :3], axis=1)
+        rj = np.linalg.norm(s["f"][mo, 3:] - fo[mo, 3:], axis=1) / np.linalg.norm(fo[mo, 3:], axis=1)
+        print(f"{label}: acc rel err max {ra.max():.3e}, jerk {rj.max():.3e}")
+        assert ra.max() <= 1e-4 and rj.max() <= 1e-3
+    assert bits_equal(s["old"][mo], f[mo])
+    rest = np.setdiff1d(np.arange(n), mo)
+    assert bits_equal(s["f"][rest], f[rest])
+    z3 = np.zeros((n, 3))
+    # The corrector is bit-identical to the oracle's on the forces the step made ...
+    a2c, a3c, rc, vc, tc, dtc = oracle.correction_pos_vel(T_SMALL, mo, po, s["f"], f, z3, z3,
+                                                          r, v, t, dt)
+    assert bits_equal(s["r"][:, :3], rc[:, :3]) and bits_equal(s["v"][:, :3], vc[:, :3])
+    assert bits_equal(s["a2"][mo], a2c[mo]) and bits_equal(s["a3"][mo], a3c[mo])
+    assert bits_equal(s["t"], tc) and bits_equal(s["dt"], dtc)
+    # ... and the new block steps are those of the oracle's own forces.
+    _, _, _, _, to, dto = oracle.correction_pos_vel(T_SMALL, mo, po, fo, f, z3, z3, r, v, t, dt)
+    assert bits_equal(s["t"], to) and bits_equal(s["dt"], dto)
+    assert atime == oracle.next_integration_time(to, dto)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_small_path_loop_modes_agree(prec):
+    """N = 1000 to t = 0.25: the graph-replayed device loop, the loop
+    launched without graphs and one h4h_step call per block step give the
+    same schedule, bit-identical state and the same number of one-workgroup
+    steps; the loop honours max_steps and the output threshold."""
+    r, v = ic.plummer(1000, 321)
+    out = {}
+    for mode, graphs in (("run", True), ("step", True), ("run", False)):
+        with Hermite4HIP(r, v, E2, precision=prec) as h:
+            h.set_graphs(graphs)
+            res = h.integration(0.25, mode=mode)
+            out[mode, graphs] = (res, h.download(), h.stats())
+    (ra, sa, sta) = out["run", True]
+    n_steps = len(ra["step_time"])
+    print(f"N=1000 {prec}: {n_steps} block steps, {sta['small_steps']} in one workgroup")
+    assert 0 < sta["small_steps"] < n_steps
+    for key in (("step", True), ("run", False)):
+        rb, sb, stb = out[key]
+        assert np.array_equal(ra["step_time"], rb["step_time"]), key
+        assert np.array_equal(ra["step_nact"], rb["step_nact"]), key
+        assert bits_equal(sa["r"], sb["r"]) and bits_equal(sa["dt"], sb["dt"]), key
+        assert bits_equal(sa["f"], sb["f"]), key
+        np.testing.assert_array_equal(ra["energy"], rb["energy"])
+        assert stb["small_steps"] == sta["small_steps"], key
+    with Hermite4HIP(r, v, E2, precision=prec) as h:
+        h.init_acc_jrk()
+        atime = h.init_dt()
+        info, st, sn = h.run(0.0, atime, 0.25, 1.0, 5)
+        assert info["steps"] == 5 and info["stopped"] == 3
+        assert np.array_equal(st, ra["step_time"][:5]) and np.array_equal(sn, ra["step_nact"][:5])
+        info, st, _ = h.run(info["itime"], info["atime"], 0.25, 0.0625, 1 << 16)
+        assert info["stopped"] == 1 and st[-1] >= 0.0625 and np.all(st[:-1] < 0.0625)
+        assert np.array_equal(st, ra["step_time"][5:5 + len(st)])
+
+
+@pytest.mark.parametrize("n", [1000, 1024])
+def test_small_path_matches_multi_workgroup_path(n):
+    """FP64 to t = 0.25 with the one-workgroup step on and off: the same
+    block-step schedule, and states and energies that differ only by the
+    order of the pair sums."""
+    r, v = ic.plummer(n, 4321 + n)
+    out = {}
+    for max_active in (256, 0):
+        with Hermite4HIP(r, v, E2, precision="fp64") as h:
+            h.set_small_path(max_active)
+            res = h.integration(0.25)
+            out[max_active] = (res, h.download(), h.stats())
+    (ra, sa, sta), (rb, sb, stb) = out[256], out[0]
+    print(f"N={n}: {len(ra['step_time'])} block steps, {sta['small_steps']} in one workgroup")
+    assert sta["small_steps"] > 0 and stb["small_steps"] == 0
+    assert np.array_equal(ra["step_time"], rb["step_time"])
+    assert np.array_equal(ra["step_nact"], rb["step_nact"])
+    assert bits_equal(sa["t"], sb["t"]) and bits_equal(sa["dt"], sb["dt"])
+    np.testing.assert_allclose(sa["r"][:, :3], sb["r"][:, :3], rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(sa["v"][:, :3], sb["v"][:, :3], rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(ra["energy"], rb["energy"], rtol=1e-12)
