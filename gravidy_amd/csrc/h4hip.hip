@@ -51,6 +51,10 @@ int fail(int code, const std::string &msg) {
 #ifndef H4_MIXED_UNROLL
 #define H4_MIXED_UNROLL 2
 #endif
+// Waves of the one-workgroup step's pair loop for active sets under 64.
+#ifndef H4_SMALL_NARROW_WAVES
+#define H4_SMALL_NARROW_WAVES 8
+#endif
 
 constexpr uint32_t DEFAULT_GRID = 2048;       // persistent workgroups
 constexpr uint32_t DEFAULT_MAX_SPLIT = 1024;  // largest j split
@@ -1224,10 +1228,16 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
         }
         __syncthreads();
 
-        // Pair sums of the active particles.
+        // Pair sums of the active particles on PT threads: all SB when
+        // P >= 64 (one slot per lane), H4_SMALL_NARROW_WAVES waves when
+        // P < 64 (the same FP64 work per SIMD with fewer cross-lane and
+        // cross-wave sums).
         const uint32_t lp = nact > 1 ? 32u - (uint32_t)__clz(nact - 1) : 0u;
-        const uint32_t P = 1u << lp, G = SB >> lp;
+        const uint32_t P = 1u << lp;
+        const uint32_t PT = P < 64 ? 64u * H4_SMALL_NARROW_WAVES : SB;
+        const uint32_t G = PT >> lp;
         const uint32_t slot = tid & (P - 1), g = tid >> lp;
+        const bool pt = tid < PT;  // (PT is a multiple of 64: wave-uniform)
         const uint32_t i = lact[slot < nact ? slot : nact - 1];
         double pi[6];
         {
@@ -1240,7 +1250,8 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
             pi[5] = b.z;
         }
         double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if constexpr (!MIXED) {
+        if (!pt) {
+        } else if constexpr (!MIXED) {
             for (uint32_t j = g; j < n; j += G) {
                 const double4 a = lpp[j], b = lpv[j];
                 const bool me = j == i;
@@ -1261,13 +1272,13 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
                 for (int c = 0; c < 6; ++c) fo[c] += (double)t6[c];
             }
         }
-        if (P < 64)
+        if (P < 64 && pt)
             for (uint32_t o = P; o < 64; o <<= 1)
                 for (int c = 0; c < 6; ++c) fo[c] += __shfl_xor(fo[c], (int)o);
         // P <= 64: wave w holds the sums of slots 0..P-1 in lanes 0..P-1
         // (group w); P > 64: every lane holds one slot, group g.
-        const uint32_t ngrp = P <= 64 ? SB / 64 : G;
-        if (P > 64 || lane < P) {
+        const uint32_t ngrp = P <= 64 ? PT / 64 : G;
+        if (pt && (P > 64 || lane < P)) {
             const uint32_t grp = P <= 64 ? w : g;
             for (int c = 0; c < 6; ++c) red[(grp * P + slot) * 6 + c] = fo[c];
         }
