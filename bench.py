@@ -146,13 +146,14 @@ def cpu_baseline(r, v, e2, sample_nact, seed=7):
             "seconds": el}
 
 
-def run_leg(e2, n, seed, t_end, precision, cpu=False):
+def run_leg(e2, n, seed, t_end, precision, cpu=False, graphs=True):
     """A whole integration (device loop) of a seeded Plummer sphere: block
     steps, interactions and the rate; with cpu=True the oracle runs the same
     integration on the host cores and its time is reported beside it."""
     from gravidy_amd import Hermite4HIP, ic
     r, v = ic.plummer(n, seed)
     with Hermite4HIP(r, v, e2, precision=precision) as h:
+        h.set_graphs(graphs)
         t0 = time.perf_counter()
         run = h.integration(t_end)
         el = time.perf_counter() - t0
@@ -166,6 +167,7 @@ def run_leg(e2, n, seed, t_end, precision, cpu=False):
     if small:
         # the same run with every block step on the multi-workgroup kernels
         with Hermite4HIP(r, v, e2, precision=precision) as h:
+            h.set_graphs(graphs)
             h.set_small_path(0)
             t0 = time.perf_counter()
             h.integration(t_end)
@@ -411,9 +413,10 @@ def main():
     runs = None
     if not args.no_runs:
         log("[bench] whole runs: Plummer N=1024 to t=1 (+ CPU oracle), N=65536 to t=0.5")
+        g = not args.no_graphs
         runs = {"plummer_n1024_t1": run_leg(e2, 1024, 12345, 1.0, args.precision,
-                                            cpu=args.cpu_sample > 0),
-                "plummer_n65536_t0.5": run_leg(e2, 65536, 65536, 0.5, args.precision)}
+                                            cpu=args.cpu_sample > 0, graphs=g),
+                "plummer_n65536_t0.5": run_leg(e2, 65536, 65536, 0.5, args.precision, graphs=g)}
 
     cpu = None
     if args.cpu_sample > 0:
