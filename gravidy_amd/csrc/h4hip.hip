@@ -1164,7 +1164,8 @@ __device__ __forceinline__ double small_min(double v, double *wmin) {
 // before its predictor; when it is empty or larger than max_act the launch
 // ends and leaves that step to the multi-workgroup kernels.
 //
-// Pair sums: P = nact rounded up to a power of two i slots, G = SB / P lanes
+// Pair sums: P = nact rounded up to a power of two i slots over PT threads
+// (all SB for P >= 64, H4_SMALL_NARROW_WAVES waves below), G = PT / P lanes
 // per slot (lane g of slot p takes j = g, g + G, ...; P >= 64 puts one j per
 // wave, an LDS broadcast), reduced by shuffles inside a wave and in wave
 // order through LDS: a fixed order for a given nact.  The mixed path folds

@@ -145,3 +145,18 @@ def test_small_path_matches_multi_workgroup_path(n):
     np.testing.assert_allclose(sa["r"][:, :3], sb["r"][:, :3], rtol=1e-10, atol=1e-13)
     np.testing.assert_allclose(sa["v"][:, :3], sb["v"][:, :3], rtol=1e-10, atol=1e-13)
     np.testing.assert_allclose(ra["energy"], rb["energy"], rtol=1e-12)
+
+
+def test_exact_precision_keeps_the_multi_workgroup_path(oracle):
+    """FP64_EXACT is bit-identical to the oracle only with the exact update
+    kernel: its steps never go through the one-workgroup path."""
+    r, v = ic.plummer(512, 99)
+    with Hermite4HIP(r, v, E2, precision="exact") as h:
+        h.set_small_path(256)
+        run = h.integration(0.125)
+        st = h.stats()
+        s = h.download()
+    ref = oracle.integrate(r, v, E2, 0.125)
+    assert st["small_steps"] == 0
+    assert np.array_equal(run["step_time"], ref["step_time"])
+    assert bits_equal(s["r"], ref["r"])
