@@ -1718,7 +1718,7 @@ int launch_plan(h4h_ctx *ctx) {
     return H4H_OK;
 }
 
-// Whether the device can launch k_small_run (SB threads, 131 KiB of LDS).
+// Whether the device can launch k_small_run (SB threads, 130 KiB of LDS).
 bool small_fits(int device) {
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)

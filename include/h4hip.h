@@ -300,7 +300,7 @@ int h4h_set_launch_choice(h4h_ctx *ctx, const h4h_launch_choice *choice);
  * the multi-workgroup kernels.  Default max_active 64; values above 256 act
  * as 256; 0 turns the path off.  Same algorithm, same block-step schedule;
  * the pair sums are added in another fixed order.  Ignored where the device
- * cannot launch a 1024-thread workgroup with 131 KiB of LDS.  No
+ * cannot launch a 1024-thread workgroup with 130 KiB of LDS.  No
  * counterpart in the reference (its GPU path launches kernels per step). */
 int h4h_set_small_path(h4h_ctx *ctx, uint32_t max_active);
 
