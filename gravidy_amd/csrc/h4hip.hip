@@ -1111,6 +1111,8 @@ __global__ __launch_bounds__(BS) void k_step_end(Ens e) {
 constexpr uint32_t SB = 1024;         // threads of k_small_run, and the largest N it takes
 constexpr uint32_t SMALL_MAX_ACT = 256;  // largest active set k_small_run takes
 static_assert(SB % JT == 0 && SB / JT <= SB / 64, "block minima from the wave minima");
+static_assert(H4_SMALL_NARROW_WAVES >= 1 && H4_SMALL_NARROW_WAVES <= SB / 64,
+              "pair waves of the one-workgroup step");
 
 // p (a global-memory pointer), as a value the compiler cannot hoist out of
 // a loop: stores through it inside the loop address SGPR base + VGPR offset
