@@ -1153,8 +1153,10 @@ __device__ __forceinline__ double small_min(double v, double *wmin) {
     return m;
 }
 
-// Block steps of a single system of n <= SB particles in one workgroup of SB
-// threads, up to max_run steps per launch.  Thread i owns particle i and
+// Block steps of systems of n <= SB particles, workgroup c of SB threads for
+// cluster c (larger clusters' workgroups return at once), up to max_run
+// steps per launch; clusters are independent, so the workgroups never wait
+// for each other.  Thread i owns particle i of its cluster and
 // keeps its position, velocity, forces, time and step in registers for the
 // whole launch; the threads exchange the predictor records and the active
 // list through LDS, and the state goes to HBM as it changes (stores
@@ -1187,8 +1189,11 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
     __shared__ double wmin[SB / 64];
 
     const uint32_t tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
-    ClState *cs = &e.cs[0];
-    const uint32_t n = e.cnt[0];
+    const uint32_t c = blockIdx.x;
+    ClState *cs = &e.cs[c];
+    const uint32_t n = e.cnt[c];
+    if (n > SB) return;
+    const uint32_t off = e.off[c];  // first particle of the cluster
     const bool in = tid < n;
     if (max_act > SMALL_MAX_ACT) max_act = SMALL_MAX_ACT;
     if (cs->done) return;
@@ -1202,14 +1207,15 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
     double ti = 0.0, dti = 0.0;
     float mf = 0.0f;
     if (in) {
-        for (int c = 0; c < 3; ++c) {
-            r[c] = s.r[c][tid];
-            v[c] = s.v[c][tid];
+        const uint32_t gi = off + tid;
+        for (int k = 0; k < 3; ++k) {
+            r[k] = s.r[k][gi];
+            v[k] = s.v[k][gi];
         }
-        for (int c = 0; c < 6; ++c) f[c] = s.f[c][tid];
-        ti = s.t[tid];
-        dti = s.dt[tid];
-        mf = (float)s.m[tid];
+        for (int k = 0; k < 6; ++k) f[k] = s.f[k][gi];
+        ti = s.t[gi];
+        dti = s.dt[gi];
+        mf = (float)s.m[gi];
     }
     bool act = in && fabs(T - (ti + dti)) < 2 * DBL_EPSILON;
     uint32_t k_act;  // this particle's place in the active list
@@ -1226,7 +1232,7 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
         }
         if (act) {
             lact[k_act] = tid;
-            for (int c = 0; c < 6; ++c) stg[k_act * 7 + c] = f[c];
+            for (int k = 0; k < 6; ++k) stg[k_act * 7 + k] = f[k];
             stg[k_act * 7 + 6] = dti;
         }
         __syncthreads();
@@ -1272,18 +1278,18 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
                     pair_mixed(pi, me ? pi[0] + 1.0 : a.x, a.y, a.z, b.x, b.y, b.z,
                                me ? 0.f : (float)a.w, e2f, t6);
                 }
-                for (int c = 0; c < 6; ++c) fo[c] += (double)t6[c];
+                for (int k = 0; k < 6; ++k) fo[k] += (double)t6[k];
             }
         }
         if (P < 64 && pt)
             for (uint32_t o = P; o < 64; o <<= 1)
-                for (int c = 0; c < 6; ++c) fo[c] += __shfl_xor(fo[c], (int)o);
+                for (int k = 0; k < 6; ++k) fo[k] += __shfl_xor(fo[k], (int)o);
         // P <= 64: wave w holds the sums of slots 0..P-1 in lanes 0..P-1
         // (group w); P > 64: every lane holds one slot, group g.
         const uint32_t ngrp = P <= 64 ? PT / 64 : G;
         if (pt && (P > 64 || lane < P)) {
             const uint32_t grp = P <= 64 ? w : g;
-            for (int c = 0; c < 6; ++c) red[(grp * P + slot) * 6 + c] = fo[c];
+            for (int k = 0; k < 6; ++k) red[(grp * P + slot) * 6 + k] = fo[k];
         }
         __syncthreads();
 
@@ -1307,10 +1313,11 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
                 fn[5] = use ? fn[5] + x2.y : fn[5];
             }
             if (!MIXED)
-                for (int c = 0; c < 6; ++c) fn[c] *= PAIR_SCALE;
-            const uint32_t ia = lact[tid];
+                for (int k = 0; k < 6; ++k) fn[k] *= PAIR_SCALE;
+            const uint32_t ia = lact[tid];  // within the cluster
+            const uint32_t gi = off + ia;
             double o[6], pr[6], a2[3], a3[3];
-            for (int c = 0; c < 6; ++c) o[c] = stg[tid * 7 + c];
+            for (int k = 0; k < 6; ++k) o[k] = stg[tid * 7 + k];
             {
                 const double4 a = lpp[ia], b = lpv[ia];
                 pr[0] = a.x;
@@ -1322,36 +1329,36 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
             }
             dtn = correct_vals(o, fn, pr, stg[tid * 7 + 6], T, eta, dt_min, dt_max, rn, vn, a2,
                                a3);
-            for (int c = 0; c < 6; ++c) {
-                in_loop(s.old[c])[ia] = o[c];
-                in_loop(s.f[c])[ia] = fn[c];
+            for (int k = 0; k < 6; ++k) {
+                in_loop(s.old[k])[gi] = o[k];
+                in_loop(s.f[k])[gi] = fn[k];
             }
-            for (int c = 0; c < 3; ++c) {
-                in_loop(s.r[c])[ia] = rn[c];
-                in_loop(s.v[c])[ia] = vn[c];
-                in_loop(s.a2[c])[ia] = a2[c];
-                in_loop(s.a3[c])[ia] = a3[c];
+            for (int k = 0; k < 3; ++k) {
+                in_loop(s.r[k])[gi] = rn[k];
+                in_loop(s.v[k])[gi] = vn[k];
+                in_loop(s.a2[k])[gi] = a2[k];
+                in_loop(s.a3[k])[gi] = a3[k];
             }
-            in_loop(s.t)[ia] = T;
-            in_loop(s.dt)[ia] = dtn;
+            in_loop(s.t)[gi] = T;
+            in_loop(s.dt)[gi] = dtn;
         }
         __syncthreads();  // red is read: the corrected state goes there for the owners
         if (tid < nact) {
             double *out = red + tid * 13;
-            for (int c = 0; c < 6; ++c) out[c] = fn[c];
-            for (int c = 0; c < 3; ++c) {
-                out[6 + c] = rn[c];
-                out[9 + c] = vn[c];
+            for (int k = 0; k < 6; ++k) out[k] = fn[k];
+            for (int k = 0; k < 3; ++k) {
+                out[6 + k] = rn[k];
+                out[9 + k] = vn[k];
             }
             out[12] = dtn;
         }
         __syncthreads();
         if (act) {
             const double *in_ = red + k_act * 13;
-            for (int c = 0; c < 6; ++c) f[c] = in_[c];
-            for (int c = 0; c < 3; ++c) {
-                r[c] = in_[6 + c];
-                v[c] = in_[9 + c];
+            for (int k = 0; k < 6; ++k) f[k] = in_[k];
+            for (int k = 0; k < 3; ++k) {
+                r[k] = in_[6 + k];
+                v[k] = in_[9 + k];
             }
             ti = T;
             dti = in_[12];
@@ -1362,8 +1369,10 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
         steps += 1;
         const bool stop = T >= t_stop || steps >= max_steps || !(T < t_end) || !(mn > T);
         if (tid == 0) {
-            e.hist_t[(steps - 1) % e.hist_cap] = T;
-            e.hist_n[(steps - 1) % e.hist_cap] = nact;
+            if (c == 0) {
+                e.hist_t[(steps - 1) % e.hist_cap] = T;
+                e.hist_n[(steps - 1) % e.hist_cap] = nact;
+            }
             cs->itime = T;
             cs->atime = mn;
             cs->steps = steps;
@@ -1386,8 +1395,8 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
 
     // Predictor records of the last step made; per-block minima of t + dt.
     if (it > 0 && in) {
-        s.pp[tid] = lpp[tid];
-        s.pv[tid] = lpv[tid];
+        s.pp[off + tid] = lpp[tid];
+        s.pv[off + tid] = lpv[tid];
     }
     __syncthreads();  // wmin is rewritten below
     double vb = in ? ti + dti : (double)INFINITY;
@@ -1395,10 +1404,10 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
     if (lane == 0) wmin[w] = vb;
     __syncthreads();
     constexpr uint32_t WPB = JT / 64;  // waves per block
-    if (tid < e.nblk) {
+    if (tid < ntiles(e, c)) {
         double m = wmin[tid * WPB];
         for (uint32_t k = 1; k < WPB; ++k) m = fmin(m, wmin[tid * WPB + k]);
-        e.bmin[tid] = m;
+        e.bmin[off / JT + tid] = m;
     }
 }
 
@@ -1734,15 +1743,17 @@ bool small_fits(int device) {
     return true;
 }
 
-// Device loops of this context start each step slot with k_small_run.
+// Device loops of this context start each step slot with k_small_run (some
+// cluster has at most SB particles).
 bool small_path(const h4h_ctx *ctx) {
-    return ctx->small_ok && ctx->small_act > 0 && ctx->K == 1 && ctx->cnt[0] <= SB &&
-           ctx->precision != H4H_FP64_EXACT;
+    return ctx->small_ok && ctx->small_act > 0 && ctx->precision != H4H_FP64_EXACT &&
+           *std::min_element(ctx->cnt.begin(), ctx->cnt.end()) <= SB;
 }
 
+// One workgroup per cluster (those of more than SB particles return at once).
 int launch_small(h4h_ctx *ctx, uint32_t max_run) {
     auto *kern = ctx->precision == H4H_MIXED ? k_small_run<true> : k_small_run<false>;
-    hipLaunchKernelGGL(kern, dim3(1), dim3(SB), 0, ctx->stream, ctx->s, ctx->e, max_run,
+    hipLaunchKernelGGL(kern, dim3(ctx->K), dim3(SB), 0, ctx->stream, ctx->s, ctx->e, max_run,
                        ctx->small_act, ctx->e2, (double)ctx->eta, ctx->dt_min, ctx->dt_max);
     H4H_LAUNCHED();
     return H4H_OK;
@@ -2528,6 +2539,7 @@ int h4h_ensemble_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_ste
         ctx->stats.block_steps += d.steps;
         ctx->stats.interactions += d.interactions;
         ctx->stats.nsteps += d.nsteps;
+        ctx->stats.small_steps += d.small_steps;
         if (info) {
             info[c].itime = d.itime;
             info[c].atime = d.atime;

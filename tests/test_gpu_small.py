@@ -160,3 +160,36 @@ def test_exact_precision_keeps_the_multi_workgroup_path(oracle):
     assert st["small_steps"] == 0
     assert np.array_equal(run["step_time"], ref["step_time"])
     assert bits_equal(s["r"], ref["r"])
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_ensemble_of_small_clusters_matches_oracle(oracle, graphs):
+    """An ensemble whose clusters of at most 1024 particles step in their own
+    workgroups (the 1500-particle one keeps the multi-workgroup path): each
+    cluster's step count, interactions and cumulative energy error agree
+    with the oracle's, and graph replay and direct launches give the same
+    state bit for bit."""
+    from gravidy_amd import EnsembleHIP
+    sizes = [64, 300, 700, 1024, 1500]
+    systems = [ic.plummer(n, 1300 + n) for n in sizes]
+    out = {}
+    for g in (graphs, not graphs):
+        with EnsembleHIP(systems, E2, precision="fp64") as ens:
+            ens.set_graphs(g)
+            res = ens.integration(0.25)
+            out[g] = (res, ens.download(), ens.stats())
+    res, state, st = out[graphs]
+    assert st["small_steps"] > 0
+    for c, (sa, sb) in enumerate(zip(state, out[not graphs][1])):
+        for key in ("r", "v", "f", "t", "dt"):
+            assert bits_equal(sa[key], sb[key]), (c, key)
+    for c, (r, v) in enumerate(systems):
+        ref = oracle.integrate(r, v, E2, 0.25)
+        e, eo = res["energy"][:, c], ref["energy"]
+        cum = abs((e[-1] - e[0]) / e[0])
+        cum_ref = abs((eo[-1] - eo[0]) / eo[0])
+        print(f"cluster {c} n={sizes[c]}: steps {res['steps'][c]}/{len(ref['step_time'])} "
+              f"CumE {cum:.4e} oracle {cum_ref:.4e}")
+        assert res["steps"][c] == len(ref["step_time"])
+        assert res["interactions"][c] == ref["interactions"]
+        assert abs(cum - cum_ref) <= 0.05 * cum_ref + 1e-15
