@@ -241,6 +241,35 @@ def copy_probe(mib, reps=3):
             "gb_per_s": 2 * nbytes / el / 1e9}
 
 
+def small_clusters_leg(e2, precision, count=256, seed=256, t_end=0.25, graphs=True):
+    """An ensemble of many small Plummer clusters (N log-uniform in
+    [128, 1024], seeded), every cluster to t_end: the clusters' block steps
+    in workgroups of their own (the default) and, for comparison, every
+    step on the multi-workgroup kernels."""
+    from gravidy_amd import EnsembleHIP, ic
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(128), np.log(1024), count)).astype(int)
+    systems = [ic.plummer(int(nk), 5000 + k) for k, nk in enumerate(sizes)]
+    out = {"clusters": count, "particles": int(sizes.sum()), "n_range": [128, 1024],
+           "seed": seed, "t_end": t_end, "precision": precision}
+    for label, limit in (("", None), ("_multi_workgroup_only", 0)):
+        with EnsembleHIP(systems, e2, precision=precision) as ens:
+            ens.set_graphs(graphs)
+            if limit is not None:
+                ens.set_small_path(limit)
+            t0 = time.perf_counter()
+            res = ens.integration(t_end)
+            el = time.perf_counter() - t0
+            st = ens.stats()
+        out["seconds" + label] = el
+        if not label:
+            out["cluster_steps"] = int(st["block_steps"])
+            out["own_workgroup_steps"] = int(st["small_steps"])
+            out["interactions"] = int(res["interactions"].sum())
+            out["pairs_per_s"] = out["interactions"] / el
+    return out
+
+
 def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4, graphs=True,
                  tune_steps=64, launch=None):
     """64 independent Plummer clusters, N_k log-uniform in [1024, 32768]
@@ -409,6 +438,11 @@ def main():
                            launch=[int(x) for x in args.ensemble_launch.split(",")]
                            if args.ensemble_launch else None)
         log(f"[bench] ensemble {ens['pairs_per_s']:.3e} pairs/s in {ens['seconds']:.2f} s")
+        log("[bench] ensemble of 256 small clusters, N in [128, 1024], to t=0.25")
+        ens["small_clusters"] = small_clusters_leg(e2, args.precision, graphs=not args.no_graphs)
+        log(f"[bench] small clusters {ens['small_clusters']['seconds']:.3f} s "
+            f"({ens['small_clusters']['seconds_multi_workgroup_only']:.3f} s without "
+            f"own-workgroup steps)")
 
     runs = None
     if not args.no_runs:

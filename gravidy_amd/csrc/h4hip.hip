@@ -1154,7 +1154,7 @@ __device__ __forceinline__ double small_min(double v, double *wmin) {
 }
 
 // Block steps of systems of n <= SB particles, workgroup c of SB threads for
-// cluster c (larger clusters' workgroups return at once), up to max_run
+// cluster c (a larger cluster's workgroup returns at once), up to max_run
 // steps per launch; clusters are independent, so the workgroups never wait
 // for each other.  Thread i owns particle i of its cluster and
 // keeps its position, velocity, forces, time and step in registers for the
@@ -1558,7 +1558,7 @@ namespace {
 constexpr uint32_t RUN_CHUNK = 32;    // block steps per captured graph
 constexpr uint32_t SMALL_RUN = 255;   // k_small_run steps per launch
 constexpr uint32_t SMALL_SLOTS = 8;   // (k_small_run + block step) pairs per captured graph
-constexpr uint32_t SMALL_ACT = 64;    // default largest active set k_small_run takes
+constexpr uint32_t SMALL_ACT = 64;    // default largest active set k_small_run takes (one system)
 constexpr uint32_t HIST_CAP = 2048;   // per-step history ring of cluster 0
 static_assert(HIST_CAP >= RUN_CHUNK && HIST_CAP >= SMALL_SLOTS * (SMALL_RUN + 1),
               "history ring shorter than the steps of one graph");
@@ -1743,14 +1743,16 @@ bool small_fits(int device) {
     return true;
 }
 
-// Device loops of this context start each step slot with k_small_run (some
-// cluster has at most SB particles).
+// Device loops of this context start each step slot with k_small_run: every
+// cluster has at most SB particles.  (With larger clusters beside them, the
+// small clusters' runs of up to SMALL_RUN steps would hold up the larger
+// clusters' steps, which wait for the launch to end.)
 bool small_path(const h4h_ctx *ctx) {
     return ctx->small_ok && ctx->small_act > 0 && ctx->precision != H4H_FP64_EXACT &&
-           *std::min_element(ctx->cnt.begin(), ctx->cnt.end()) <= SB;
+           *std::max_element(ctx->cnt.begin(), ctx->cnt.end()) <= SB;
 }
 
-// One workgroup per cluster (those of more than SB particles return at once).
+// One workgroup per cluster.
 int launch_small(h4h_ctx *ctx, uint32_t max_run) {
     auto *kern = ctx->precision == H4H_MIXED ? k_small_run<true> : k_small_run<false>;
     hipLaunchKernelGGL(kern, dim3(ctx->K), dim3(SB), 0, ctx->stream, ctx->s, ctx->e, max_run,
@@ -2030,6 +2032,9 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
     int rc = configure_launch(ctx);
     if (rc) return cleanup(rc);
     ctx->small_ok = small_fits(ctx->device);
+    // An ensemble's other clusters keep the rest of the chip busy, so a
+    // larger step in a cluster's own workgroup costs the others nothing.
+    ctx->small_act = K > 1 ? SMALL_MAX_ACT : SMALL_ACT;
 
     double *b = ctx->state;
     Soa &s = ctx->s;

@@ -76,10 +76,11 @@ class EnsembleHIP:
         """Device loops replay captured graphs (default) or launch directly."""
         check("h4h_set_graphs", self.lib.h4h_set_graphs(self._ctx, 1 if enable else 0))
 
-    def set_small_path(self, max_active=64):
-        """Clusters of at most 1024 particles make their block steps with
-        1..max_active (at most 256) active particles in a workgroup of their
-        own (0: every step on the multi-workgroup kernels)."""
+    def set_small_path(self, max_active=256):
+        """When every cluster has at most 1024 particles, each cluster makes
+        its block steps with 1..max_active (at most 256; the default for an
+        ensemble) active particles in a workgroup of its own (0: every step
+        on the multi-workgroup kernels)."""
         check("h4h_set_small_path", self.lib.h4h_set_small_path(self._ctx, int(max_active)))
 
     def set_launch(self, workgroups=0, max_jsplit=0):

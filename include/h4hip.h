@@ -292,16 +292,21 @@ int h4h_tune_launch(h4h_ctx *ctx, uint32_t steps, h4h_launch_choice *choice);
  * run); ms_per_step is ignored.  No counterpart in the reference. */
 int h4h_set_launch_choice(h4h_ctx *ctx, const h4h_launch_choice *choice);
 
-/* Small systems (one system of at most 1024 particles, FP64 or mixed
- * precision): h4h_step and the device loop of h4h_run make every block step
- * whose active set has 1 to max_active particles in one workgroup (each
- * particle's state in one thread's registers, the predictor records and the
- * active list in LDS), up to 255 steps per launch; larger steps go through
- * the multi-workgroup kernels.  Default max_active 64; values above 256 act
- * as 256; 0 turns the path off.  Same algorithm, same block-step schedule;
- * the pair sums are added in another fixed order.  Ignored where the device
- * cannot launch a 1024-thread workgroup with 130 KiB of LDS.  No
- * counterpart in the reference (its GPU path launches kernels per step). */
+/* Small systems (of at most 1024 particles, FP64 or mixed precision):
+ * h4h_step and the device loops of h4h_run and h4h_ensemble_run make every
+ * block step whose active set has 1 to max_active particles in one
+ * workgroup (each particle's state in one thread's registers, the predictor
+ * records and the active list in LDS), up to 255 steps per launch; larger
+ * steps go through the multi-workgroup kernels.  An ensemble whose clusters
+ * all have at most 1024 particles gives each cluster a workgroup of its own
+ * (one with a larger cluster keeps the multi-workgroup kernels for all:
+ * the larger clusters would wait for the small ones' runs).  Default
+ * max_active 64 for
+ * one system, 256 for an ensemble; values above 256 act as 256; 0 turns the
+ * path off.  Same algorithm, same block-step schedule; the pair sums are
+ * added in another fixed order.  Ignored where the device cannot launch a
+ * 1024-thread workgroup with 130 KiB of LDS.  No counterpart in the
+ * reference (its GPU path launches kernels per step). */
 int h4h_set_small_path(h4h_ctx *ctx, uint32_t max_active);
 
 /* Enables HIP-event timing of update passes and steps (stats.update_ms,

@@ -164,13 +164,12 @@ def test_exact_precision_keeps_the_multi_workgroup_path(oracle):
 
 @pytest.mark.parametrize("graphs", [True, False])
 def test_ensemble_of_small_clusters_matches_oracle(oracle, graphs):
-    """An ensemble whose clusters of at most 1024 particles step in their own
-    workgroups (the 1500-particle one keeps the multi-workgroup path): each
-    cluster's step count, interactions and cumulative energy error agree
-    with the oracle's, and graph replay and direct launches give the same
-    state bit for bit."""
+    """An ensemble of clusters of at most 1024 particles, each stepping in a
+    workgroup of its own: each cluster's step count, interactions and
+    cumulative energy error agree with the oracle's, and graph replay and
+    direct launches give the same state bit for bit."""
     from gravidy_amd import EnsembleHIP
-    sizes = [64, 300, 700, 1024, 1500]
+    sizes = [64, 300, 700, 1000, 1024]
     systems = [ic.plummer(n, 1300 + n) for n in sizes]
     out = {}
     for g in (graphs, not graphs):
@@ -193,3 +192,13 @@ def test_ensemble_of_small_clusters_matches_oracle(oracle, graphs):
         assert res["steps"][c] == len(ref["step_time"])
         assert res["interactions"][c] == ref["interactions"]
         assert abs(cum - cum_ref) <= 0.05 * cum_ref + 1e-15
+
+
+def test_ensemble_with_a_large_cluster_keeps_the_multi_workgroup_path():
+    """A cluster of more than 1024 particles in the ensemble: no cluster
+    steps in a workgroup of its own (the large one would wait for them)."""
+    from gravidy_amd import EnsembleHIP
+    systems = [ic.plummer(n, 1400 + n) for n in (300, 1500)]
+    with EnsembleHIP(systems, E2, precision="fp64") as ens:
+        ens.integration(0.0625)
+        assert ens.stats()["small_steps"] == 0
