@@ -1479,24 +1479,32 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
 // Items are pairs of (i tile, j tile) with j tile >= i tile of the cluster
 // at device offset off, numbered row by row;
 // out[4*block] = {epot, epot_comp, ekin, ekin_comp}.
-__global__ __launch_bounds__(BS) void k_energy(Soa s, uint32_t off, uint32_t n, double *out) {
+// Cluster c0 + blockIdx.y of the cluster tables offs / cnts; workgroup
+// blockIdx.x takes the tile pairs blockIdx.x, + stride, + 2 stride, ...
+// (stride >= gridDim.x: a launch with fewer workgroups than stride has the
+// same per-workgroup sums as one with stride workgroups, the others' being
+// zero), and writes its four sums at out[4 (blockIdx.y gridDim.x +
+// blockIdx.x)].
+__global__ __launch_bounds__(BS) void k_energy(Soa s, const uint32_t *offs, const uint32_t *cnts,
+                                              uint32_t c0, uint32_t stride, double *out) {
 #pragma clang fp contract(off)
     __shared__ double4 sr[JT];
     __shared__ double red[2 * (BS / 64)];
+    const uint32_t off = offs[c0 + blockIdx.y], n = cnts[c0 + blockIdx.y];
     const uint32_t nt = (n + BS - 1) / BS;
     const uint64_t items = (uint64_t)nt * (nt + 1) / 2;
     double ep = 0.0, epc = 0.0, ek = 0.0, ekc = 0.0;
     // Row `it` holds nt - it items.  (it, rem) locate the current item and
-    // advance by gridDim.x per iteration, so the walk is O(nt) in total.
+    // advance by stride per iteration, so the walk is O(nt) in total.
     uint32_t it = 0;
     uint64_t rem = blockIdx.x;
-    for (uint64_t item = blockIdx.x; item < items; item += gridDim.x) {
+    for (uint64_t item = blockIdx.x; item < items; item += stride) {
         while (rem >= (uint64_t)(nt - it)) {
             rem -= (nt - it);
             ++it;
         }
         uint32_t jt = it + (uint32_t)rem;
-        rem += gridDim.x;
+        rem += stride;
         uint32_t li = it * BS + threadIdx.x;
         uint32_t lj0 = jt * JT;
         __syncthreads();
@@ -1541,10 +1549,11 @@ __global__ __launch_bounds__(BS) void k_energy(Soa s, uint32_t off, uint32_t n, 
     __syncthreads();
     block_sum2(ek, ekc, red);
     if (threadIdx.x == 0) {
-        out[4 * blockIdx.x + 0] = ep;
-        out[4 * blockIdx.x + 1] = epc;
-        out[4 * blockIdx.x + 2] = ek;
-        out[4 * blockIdx.x + 3] = ekc;
+        double *o = out + 4 * ((size_t)blockIdx.y * gridDim.x + blockIdx.x);
+        o[0] = ep;
+        o[1] = epc;
+        o[2] = ek;
+        o[3] = ekc;
     }
 }
 
@@ -1892,17 +1901,15 @@ int download_scalar(h4h_ctx *ctx, const double *src, double *dst) {
 
 bool valid_precision(int p) { return p == H4H_FP64 || p == H4H_FP64_EXACT || p == H4H_MIXED; }
 
-int energy_of(h4h_ctx *ctx, uint32_t c, double *ekin, double *epot) {
-    hipLaunchKernelGGL(k_energy, dim3(ENERGY_GRID), dim3(BS), 0, ctx->stream, ctx->s, ctx->off[c],
-                       ctx->cnt[c], ctx->eout);
-    H4H_LAUNCHED();
+// Energies of clusters c0 .. c0 + count - 1, in launches of as many
+// clusters as fit the ENERGY_GRID x 4 sums of eout: per cluster the
+// workgroups that have tile pairs (at most ENERGY_GRID, item stride
+// ENERGY_GRID as in a launch of ENERGY_GRID workgroups for that cluster
+// alone), then a host Neumaier sum of its workgroup sums in order.
+int energies_of(h4h_ctx *ctx, uint32_t c0, uint32_t count, double *ekin, double *epot) {
     std::vector<double> h(ENERGY_GRID * 4);
-    H4H_HIP(hipMemcpyAsync(h.data(), ctx->eout, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                           ctx->stream));
-    H4H_HIP(hipStreamSynchronize(ctx->stream));
     // Neumaier sum of the per-workgroup (sum, compensation) pairs (an
     // infinite sum leaves the compensation alone, as on the device).
-    double sp = 0.0, cp = 0.0, sk = 0.0, ck = 0.0;
     auto add = [](double &s, double &c2, double x) {
         double t = s + x;
         if (std::isfinite(t)) {
@@ -1913,14 +1920,38 @@ int energy_of(h4h_ctx *ctx, uint32_t c, double *ekin, double *epot) {
         }
         s = t;
     };
-    for (uint32_t b = 0; b < ENERGY_GRID; ++b) {
-        add(sp, cp, h[4 * b + 0]);
-        add(sp, cp, h[4 * b + 1]);
-        add(sk, ck, h[4 * b + 2]);
-        add(sk, ck, h[4 * b + 3]);
+    auto wgs = [&](uint32_t c) {
+        const uint64_t nt = (ctx->cnt[c] + BS - 1) / BS;
+        return (uint32_t)std::min<uint64_t>(ENERGY_GRID, nt * (nt + 1) / 2);
+    };
+    for (uint32_t b0 = c0; b0 < c0 + count;) {
+        uint32_t gx = wgs(b0), kb = 1;
+        while (b0 + kb < c0 + count) {
+            const uint32_t g2 = std::max(gx, wgs(b0 + kb));
+            if ((uint64_t)(kb + 1) * g2 > ENERGY_GRID) break;
+            gx = g2;
+            ++kb;
+        }
+        hipLaunchKernelGGL(k_energy, dim3(gx, kb), dim3(BS), 0, ctx->stream, ctx->s, ctx->d_off,
+                           ctx->d_cnt, b0, (uint32_t)ENERGY_GRID, ctx->eout);
+        H4H_LAUNCHED();
+        H4H_HIP(hipMemcpyAsync(h.data(), ctx->eout, (size_t)kb * gx * 4 * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        H4H_HIP(hipStreamSynchronize(ctx->stream));
+        for (uint32_t k = 0; k < kb; ++k) {
+            double sp = 0.0, cp = 0.0, sk = 0.0, ck = 0.0;
+            const double *hk = h.data() + (size_t)k * gx * 4;
+            for (uint32_t b = 0; b < gx; ++b) {
+                add(sp, cp, hk[4 * b + 0]);
+                add(sp, cp, hk[4 * b + 1]);
+                add(sk, ck, hk[4 * b + 2]);
+                add(sk, ck, hk[4 * b + 3]);
+            }
+            if (ekin) ekin[b0 - c0 + k] = sk + ck;
+            if (epot) epot[b0 - c0 + k] = sp + cp;
+        }
+        b0 += kb;
     }
-    if (ekin) *ekin = sk + ck;
-    if (epot) *epot = sp + cp;
     return H4H_OK;
 }
 
@@ -2519,7 +2550,7 @@ int h4h_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_steps, h4h_r
 int h4h_get_energy(h4h_ctx *ctx, double *ekin, double *epot) {
     int rc = check_single(ctx);
     if (rc) return rc;
-    return energy_of(ctx, 0, ekin, epot);
+    return energies_of(ctx, 0, 1, ekin, epot);
 }
 
 int h4h_ensemble_init(h4h_ctx *ctx, float eta_s) {
@@ -2563,13 +2594,7 @@ int h4h_ensemble_run(h4h_ctx *ctx, double t_end, double t_stop, uint64_t max_ste
 int h4h_ensemble_energy(h4h_ctx *ctx, double *ekin, double *epot) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
-    for (uint32_t c = 0; c < ctx->K; ++c) {
-        double k = 0.0, p = 0.0;
-        if ((rc = energy_of(ctx, c, &k, &p))) return rc;
-        if (ekin) ekin[c] = k;
-        if (epot) epot[c] = p;
-    }
-    return H4H_OK;
+    return energies_of(ctx, 0, ctx->K, ekin, epot);
 }
 
 int h4h_ensemble_sizes(h4h_ctx *ctx, uint32_t *nclusters, uint32_t *sizes) {
