@@ -64,4 +64,6 @@ def test_null_context_is_an_error():
     assert lib.h4h_init_acc_jrk(None) == _lib.H4H_EINVAL
     assert lib.h4h_step(None, 0.0, None, None) == _lib.H4H_EINVAL
     assert lib.h4h_get_energy(None, None, None) == _lib.H4H_EINVAL
+    assert lib.h4h_set_small_path(None, 64) == _lib.H4H_EINVAL
+    assert lib.h4h_ensemble_energy(None, None, None) == _lib.H4H_EINVAL
     assert lib.h4h_create(None, 16, 1e-8, 0.01, 2.0 ** -23, 0.125, 0) == _lib.H4H_EINVAL
