@@ -202,3 +202,27 @@ def test_ensemble_with_a_large_cluster_keeps_the_multi_workgroup_path():
     with EnsembleHIP(systems, E2, precision="fp64") as ens:
         ens.integration(0.0625)
         assert ens.stats()["small_steps"] == 0
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_small_step_without_softening(oracle, prec):
+    """e2 = 0: the pair of a particle with itself (r = 0) is skipped, so the
+    one-workgroup step stays finite and matches the oracle."""
+    n = 700
+    r, v, f, t, dt = small_state(n, 8)
+    mo = oracle.find_particles_to_move(T_SMALL, t, dt)
+    with Hermite4HIP(r, v, 0.0, precision=prec) as h:
+        h.set_small_path(256)
+        h.upload(r, v, f, t, dt)
+        h.stats(reset=True)
+        k, _ = h.step(T_SMALL)
+        assert h.stats()["small_steps"] == 1
+        s = h.download()
+    assert k == mo.size
+    po, pmo = oracle.predicted_pos_vel(T_SMALL, t, r, v, f)
+    fo = oracle.update_acc_jrk(mo, po, pmo, 0.0, f)
+    assert np.all(np.isfinite(s["f"]))
+    if prec == "fp64":
+        check_forces(s["f"], fo, mo, po, pmo, 0.0, f"small step e2=0 n={n} fp64")
+    else:
+        check_mixed(s["f"], fo, mo, f"small step e2=0 n={n} mixed")
