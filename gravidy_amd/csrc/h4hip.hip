@@ -1200,7 +1200,14 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
     const double t_end = e.ctl->t_end, t_stop = e.ctl->t_stop;
     const unsigned long long max_steps = e.ctl->max_steps;
     unsigned long long steps = cs->steps;
+    uint32_t hk = (uint32_t)(steps % e.hist_cap);  // history slot of the next step
     double T = cs->atime;
+    // With softening large enough, a particle's pair with itself (r = 0, v = 0)
+    // has finite weights and adds exact zeros, so the j loop needs no test for
+    // it: 1/r^3 <= e2^-1.5 stays below 2^1000 in FP64, and in FP32 (float)e2
+    // >= 2^-60 keeps 1/r^3 below 2^90 (an FP32 mass <= 2^24 times that is
+    // finite).
+    const bool self_zero = MIXED ? (float)e2 >= 0x1p-60f : e2 >= 0x1p-600;
 
     double r[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
     double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1261,22 +1268,36 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
         double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (!pt) {
         } else if constexpr (!MIXED) {
-            for (uint32_t j = g; j < n; j += G) {
-                const double4 a = lpp[j], b = lpv[j];
-                const bool me = j == i;
-                pair_fp64(pi, me ? pi[0] + 1.0 : a.x, a.y, a.z, b.x, b.y, b.z, me ? 0.0 : a.w, e2,
-                          fo);
+            if (self_zero) {
+                for (uint32_t j = g; j < n; j += G) {
+                    const double4 a = lpp[j], b = lpv[j];
+                    pair_fp64(pi, a.x, a.y, a.z, b.x, b.y, b.z, a.w, e2, fo);
+                }
+            } else {
+                for (uint32_t j = g; j < n; j += G) {
+                    const double4 a = lpp[j], b = lpv[j];
+                    const bool me = j == i;
+                    pair_fp64(pi, me ? pi[0] + 1.0 : a.x, a.y, a.z, b.x, b.y, b.z,
+                              me ? 0.0 : a.w, e2, fo);
+                }
             }
         } else {
             const float e2f = (float)e2;
             for (uint32_t j0 = g; j0 < n; j0 += 32 * G) {
                 const uint32_t j1 = min(n, j0 + 32 * G);
                 float t6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                for (uint32_t j = j0; j < j1; j += G) {
-                    const double4 a = lpp[j], b = lpv[j];
-                    const bool me = j == i;
-                    pair_mixed(pi, me ? pi[0] + 1.0 : a.x, a.y, a.z, b.x, b.y, b.z,
-                               me ? 0.f : (float)a.w, e2f, t6);
+                if (self_zero) {
+                    for (uint32_t j = j0; j < j1; j += G) {
+                        const double4 a = lpp[j], b = lpv[j];
+                        pair_mixed(pi, a.x, a.y, a.z, b.x, b.y, b.z, (float)a.w, e2f, t6);
+                    }
+                } else {
+                    for (uint32_t j = j0; j < j1; j += G) {
+                        const double4 a = lpp[j], b = lpv[j];
+                        const bool me = j == i;
+                        pair_mixed(pi, me ? pi[0] + 1.0 : a.x, a.y, a.z, b.x, b.y, b.z,
+                                   me ? 0.f : (float)a.w, e2f, t6);
+                    }
                 }
                 for (int k = 0; k < 6; ++k) fo[k] += (double)t6[k];
             }
@@ -1370,8 +1391,8 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
         const bool stop = T >= t_stop || steps >= max_steps || !(T < t_end) || !(mn > T);
         if (tid == 0) {
             if (c == 0) {
-                e.hist_t[(steps - 1) % e.hist_cap] = T;
-                e.hist_n[(steps - 1) % e.hist_cap] = nact;
+                e.hist_t[hk] = T;
+                e.hist_n[hk] = nact;
             }
             cs->itime = T;
             cs->atime = mn;
@@ -1385,6 +1406,7 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
             }
         }
         T = mn;
+        if (++hk == e.hist_cap) hk = 0;
         if (stop) {
             ++it;
             break;
