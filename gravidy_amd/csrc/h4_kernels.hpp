@@ -279,7 +279,7 @@ __device__ __forceinline__ void pair_exact(const double pi[6], double xj,
 constexpr double PAIR_SCALE = 0.19245008972987526;
 
 // sqrt(3) / sqrt(x) for x > 0: hardware estimate y = (1 + d) / sqrt(x)
-// (v_rsq_f64; |d| < 2^-24 over the FP64 range, measured by valu-probe),
+// (v_rsq_f64; |d| < 2^-24 over the FP64 range, measured on the part over 4M inputs),
 // then one Newton step sqrt(3) y (1 + e/2), e = 1 - x y^2, which leaves a
 // relative error of about -1.5 d^2, below 2^-47.
 __device__ __forceinline__ double rsqrt3_fp64(double x) {
