@@ -1179,7 +1179,8 @@ __device__ __forceinline__ double small_min(double v, double *wmin) {
 template <bool MIXED>
 __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run,
                                                   uint32_t max_act, double e2, double eta,
-                                                  double dt_min, double dt_max) {
+                                                  double dt_min, double dt_max,
+                                                  uint32_t self_zero) {
 #pragma clang fp contract(off)
     __shared__ double4 lpp[SB], lpv[SB];  // predictor records
     __shared__ __align__(16) double red[6 * SB];  // pair sums per (group, slot); corrector output
@@ -1202,12 +1203,9 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
     unsigned long long steps = cs->steps;
     uint32_t hk = (uint32_t)(steps % e.hist_cap);  // history slot of the next step
     double T = cs->atime;
-    // With softening large enough, a particle's pair with itself (r = 0, v = 0)
-    // has finite weights and adds exact zeros, so the j loop needs no test for
-    // it: 1/r^3 <= e2^-1.5 stays below 2^1000 in FP64, and in FP32 (float)e2
-    // >= 2^-60 keeps 1/r^3 below 2^90 (an FP32 mass <= 2^24 times that is
-    // finite).
-    const bool self_zero = MIXED ? (float)e2 >= 0x1p-60f : e2 >= 0x1p-600;
+    // self_zero (set by the host, self_pair_is_zero): a particle's pair with
+    // itself (r = 0, v = 0) has a finite weight m / e2^1.5 for every mass of
+    // the context, so it adds exact zeros and the j loop needs no test for it.
 
     double r[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
     double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1616,6 +1614,7 @@ struct h4h_ctx {
     bool small_ok = false;   // the device can launch k_small_run
     uint32_t small_act = SMALL_ACT;  // k_small_run takes active sets up to this (0: off)
     uint32_t nact_host = 0;  // size of cluster 0's active list (single-system calls)
+    double mass_max = 0.0;   // largest |float mass| of the uploaded state (+inf: not finite)
 
     hipStream_t stream = nullptr;
     double *state = nullptr;     // all SoA arrays
@@ -1783,11 +1782,26 @@ bool small_path(const h4h_ctx *ctx) {
            *std::max_element(ctx->cnt.begin(), ctx->cnt.end()) <= SB;
 }
 
+// Whether the pair loop of `precision` may add a particle's pair with itself
+// (r = 0, v = 0) untested: its weight 3 sqrt(3) m / r^3 at r^2 = e2 (FP64;
+// m / r^3 in FP32 for the mixed kernel) must be finite for the largest
+// float mass mmax of the context, and e2 inside the range where the
+// refined hardware rsqrt keeps its accuracy.  Then the pair's terms are
+// finite times zero, exact zeros.  mmax is +inf when a mass is not finite.
+bool self_pair_is_zero(int precision, double e2, double mmax) {
+    if (precision == H4H_MIXED) {
+        const float e2f = (float)e2;
+        return e2f >= 0x1p-60f && mmax * std::pow((double)e2f, -1.5) < 0x1p120;
+    }
+    return e2 >= 0x1p-600 && mmax * std::pow(e2, -1.5) < 0x1p1016;
+}
+
 // One workgroup per cluster.
 int launch_small(h4h_ctx *ctx, uint32_t max_run) {
     auto *kern = ctx->precision == H4H_MIXED ? k_small_run<true> : k_small_run<false>;
     hipLaunchKernelGGL(kern, dim3(ctx->K), dim3(SB), 0, ctx->stream, ctx->s, ctx->e, max_run,
-                       ctx->small_act, ctx->e2, (double)ctx->eta, ctx->dt_min, ctx->dt_max);
+                       ctx->small_act, ctx->e2, (double)ctx->eta, ctx->dt_min, ctx->dt_max,
+                       self_pair_is_zero(ctx->precision, ctx->e2, ctx->mass_max) ? 1u : 0u);
     H4H_LAUNCHED();
     return H4H_OK;
 }
@@ -2192,6 +2206,18 @@ int h4h_upload(h4h_ctx *ctx, const double *r4, const double *v4, const double *f
     const Soa &s = ctx->s;
     // Zero everything first (init_data semantics), then fill.
     H4H_HIP(hipMemsetAsync(ctx->state, 0, (size_t)ctx->n_dev * N_SOA * sizeof(double), ctx->stream));
+    // Largest pair weight's mass (Predictor::m is the float mass): it decides
+    // whether the one-workgroup step may skip its self-pair test, a kernel
+    // argument of the captured graphs.
+    double mmax = 0.0;
+    for (uint32_t i = 0; i < ctx->n; ++i) {
+        const double m = std::fabs((double)(float)r4[4 * (size_t)i + 3]);
+        mmax = std::isfinite(m) ? std::max(mmax, m) : (double)INFINITY;
+    }
+    if (self_pair_is_zero(ctx->precision, ctx->e2, mmax) !=
+        self_pair_is_zero(ctx->precision, ctx->e2, ctx->mass_max))
+        drop_graph(ctx);
+    ctx->mass_max = mmax;
     Cols c{};
     c.c[0] = s.r[0]; c.c[1] = s.r[1]; c.c[2] = s.r[2]; c.c[3] = s.m;
     if ((rc = upload_cols(ctx, r4, 4, c))) return rc;
