@@ -124,8 +124,8 @@ class EnsembleHIP:
     def integration(self, t_end, interval=0.125, eta_s=ETA_DEFAULT):
         """Every cluster runs the reference's loop to t_end with outputs at
         interval * k (float arithmetic as there).  Returns per-cluster
-        energies at t = 0 and at each output time, step counts and
-        interactions."""
+        energies at t = 0 and at each output time, step counts,
+        interactions and the block time of each cluster's last step."""
         t_end32 = float(np.float32(t_end))
         iv32 = np.float32(interval)
         self.init(eta_s)
@@ -134,12 +134,14 @@ class EnsembleHIP:
         times = [0.0]
         steps = np.zeros(self.K, np.int64)
         inter = np.zeros(self.K, np.int64)
+        itime = np.zeros(self.K)
         k = 1
         while True:
             t_stop = float(np.float32(iv32 * np.float32(k)))
             info = self.run(t_end32, t_stop)
             steps += [x["steps"] for x in info]
             inter += [x["interactions"] for x in info]
+            itime = np.array([x["itime"] if x["steps"] else itime[c] for c, x in enumerate(info)])
             if all(x["steps"] == 0 for x in info):
                 break
             if any(x["stopped"] == _lib.H4H_STOP_OUTPUT for x in info):
@@ -150,4 +152,4 @@ class EnsembleHIP:
             if all(x["itime"] >= t_end32 for x in info):
                 break
         return {"energy": np.array(energies), "energy_time": np.array(times),
-                "steps": steps, "interactions": inter}
+                "steps": steps, "interactions": inter, "itime": itime}

@@ -5,11 +5,21 @@ the oracle on injected states (each block step starts both sides from the
 oracle's state), from the initial conditions and again from a state the
 device loop reached on its own; energies of the same states to 1e-13.
 
+Plummer N=262144: active samples of the update against the oracle (FP64
+within 1e-13, mixed within its contract), and the mixed path's energy drift
+over the whole run to t=0.25.
+
 Salpeter masses + central particle at reduced N: Salpeter masses on [0.1, 100] (total 1) in a
 Plummer sphere plus a central particle of mass 0.01 at rest, eps=1e-5,
 block steps down to 2^-30: the exact path bit-identical to the oracle over
 a whole run (schedule, state, energies), the FP64 path with the same
 schedule and the same cumulative energy error to 2 significant digits.
+
+The ensemble recipe: 64 seeded clusters, N log-uniform in [1024, 32768],
+with the launch picked by the autotuner, each cluster checked by
+properties and the smallest ones against the oracle.
+
+Restarts from a binary checkpoint against the oracle's uninterrupted run.
 """
 import numpy as np
 import pytest
@@ -228,3 +238,119 @@ def test_python_restart_is_bit_exact(tmp_path, precision):
         assert bits_equal(sr[key], sw[key]), key
     assert second["energy_ini"] == whole["energy_ini"]
     assert np.array_equal(second["energy"][1:], whole["energy"][3:])
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_plummer262144_update_sample_against_oracle(oracle, precision):
+    """Plummer N=262144 (the benchmark's IC): 256 seeded active particles
+    against all 262144 j, FP64 within 1e-13 (two-level), mixed within its
+    RMS contract and per-particle caps."""
+    n = 262144
+    r, v = ic.plummer(n, 262144)
+    rng = np.random.default_rng(2621)
+    move = np.sort(rng.choice(n, size=256, replace=False)).astype(np.uint32)
+    with Hermite4HIP(r, v, E2, precision=precision) as h:
+        h.set_active(move)
+        h.update_acc_jrk(move.size)
+        f = h.download()["f"]
+    fo = oracle.update_acc_jrk(move, p6(r, v), pm_of(r), E2, np.zeros((n, 6)))
+    label = f"Plummer n={n} nact=256 {precision}"
+    if precision == "fp64":
+        check_forces(f, fo, move, p6(r, v), pm_of(r), E2, label)
+    else:
+        check_mixed(f, fo, move, label)
+
+
+C4_COUNT, C4_SEED = 64, 64
+
+
+@pytest.mark.slow
+def test_ensemble_recipe_64_clusters(oracle):
+    """The ensemble configuration's own recipe: 64 Plummer clusters with N
+    log-uniform in [1024, 32768] from seed 64 (cluster k from seed
+    64000 + k), FP64, the launch picked by h4h_tune_launch, integrated to
+    t = 0.125.  Every cluster reaches t_end with finite energies; graph
+    replay and direct launches of the same launch give bit-identical
+    states; the 4 smallest clusters take the oracle's number of block
+    steps and interactions and reach its cumulative energy error to 2
+    significant digits (each cluster is one run of
+    Hermite4CPU_integration.cpp:38-130)."""
+    from gravidy_amd import EnsembleHIP
+    sizes = ic.log_uniform_sizes(C4_COUNT, C4_SEED)
+    assert sizes.min() >= 1024 and sizes.max() <= 32768
+    systems = [ic.plummer(int(nk), C4_SEED * 1000 + k) for k, nk in enumerate(sizes)]
+    t_end = 0.125
+    out = {}
+    choice = None
+    for graphs in (True, False):
+        with EnsembleHIP(systems, E2, precision="fp64") as ens:
+            ens.set_graphs(graphs)
+            if choice is None:
+                ens.init()
+                choice = ens.tune_launch(16)
+                print(f"ensemble launch: {choice}")
+            else:
+                ens.set_launch_choice(choice["workgroups"], choice["max_jsplit"],
+                                      choice["iw_fill16"])
+            res = ens.integration(t_end)
+            out[graphs] = (res, ens.download())
+    res, states = out[True]
+    assert list(res["energy_time"]) == [0.0, t_end]
+    assert np.all(np.isfinite(res["energy"]))
+    assert np.all(res["itime"] == t_end), res["itime"]
+    for c, st in enumerate(states):
+        assert np.all(st["t"] <= t_end) and np.all(st["t"] + st["dt"] > t_end), c
+    res2, states2 = out[False]
+    assert np.array_equal(res["steps"], res2["steps"])
+    assert np.array_equal(res["energy"], res2["energy"])
+    for c, (a, b) in enumerate(zip(states, states2)):
+        for key in ("r", "v", "f", "t", "dt"):
+            assert bits_equal(a[key], b[key]), (c, key)
+    cum = np.abs((res["energy"][-1] - res["energy"][0]) / res["energy"][0])
+    print(f"ensemble recipe: {int(sizes.sum())} particles, block steps "
+          f"{int(res['steps'].min())}..{int(res['steps'].max())}, CumE median "
+          f"{np.median(cum):.3e} max {cum.max():.3e}")
+    for c in np.argsort(sizes)[:4]:
+        r, v = systems[c]
+        ref = oracle.integrate(r, v, E2, t_end)
+        e = ref["energy"]
+        cum_ref = abs((e[-1] - e[0]) / e[0])
+        print(f"cluster {c} n={sizes[c]}: steps {res['steps'][c]}/{len(ref['step_time'])} "
+              f"interactions {res['interactions'][c]}/{ref['interactions']} "
+              f"CumE {cum[c]:.4e} oracle {cum_ref:.4e}")
+        assert res["steps"][c] == len(ref["step_time"]), c
+        assert res["interactions"][c] == ref["interactions"], c
+        assert float(f"{cum[c]:.1e}") == float(f"{cum_ref:.1e}") or \
+            abs(cum[c] - cum_ref) <= 0.05 * cum_ref, c
+
+
+def test_python_restart_exact_matches_oracle(oracle, tmp_path):
+    """Exact precision, N = 1024 to t = 0.25 with a checkpoint, then a new
+    context restarted from it to t = 0.5: the two runs' schedules together,
+    the final state (bit for bit) and the energies at every output time
+    (1e-13) are those of the oracle's uninterrupted run to t = 0.5
+    (Hermite4CPU_integration.cpp:38-130)."""
+    r, v = ic.plummer(1024, 6161)
+    ref = oracle.integrate(r, v, E2, 0.5)
+    ck = str(tmp_path / "quarter.ckpt")
+    with Hermite4HIP(r, v, E2, precision="exact") as h:
+        first = h.integration(0.25, checkpoint=ck)
+    with Hermite4HIP(r, v, E2, precision="exact") as h:
+        second = h.integration(0.5, restart=ck)
+        s = h.download()
+    steps = np.concatenate([first["step_time"], second["step_time"]])
+    nact = np.concatenate([first["step_nact"], second["step_nact"]])
+    assert np.array_equal(steps, ref["step_time"])
+    assert np.array_equal(nact, ref["step_nact"])
+    # a restarted run carries the checkpoint's counters on
+    assert second["interactions"] == ref["interactions"]
+    assert bits_equal(s["r"], ref["r"]) and bits_equal(s["v"][:, :3], ref["v"][:, :3])
+    assert bits_equal(s["t"], ref["t"]) and bits_equal(s["dt"], ref["dt"])
+    assert bits_equal(s["f"], ref["f"])
+    times = np.concatenate([first["energy_time"], second["energy_time"][1:]])
+    energies = np.concatenate([first["energy"], second["energy"][1:]])
+    assert np.array_equal(times, ref["energy_time"])
+    np.testing.assert_allclose(energies, ref["energy"], rtol=1e-13)
+    assert second["energy_ini"] == first["energy_ini"]
+    np.testing.assert_allclose(second["energy_ini"], ref["energy"][0], rtol=1e-13)

@@ -130,3 +130,82 @@ def test_driver_restart_from_checkpoint_is_bit_exact(tmp_path, precision):
                            "--restart", str(tmp_path / "half.ckpt")],
                           capture_output=True, text=True, timeout=120)
     assert proc.returncode != 0
+
+
+def test_driver_restart_exact_matches_oracle(oracle, tmp_path):
+    """gravidy-hip, exact precision: a run to t=0.25 with a binary
+    checkpoint, continued from it to t=0.5, ends in the state of the
+    oracle's uninterrupted run to t=0.5 bit for bit (r, v, forces, t, dt
+    from the final checkpoint), and the energy rows logged after the
+    restart (t = 0.375, 0.5) carry the oracle's energies, iteration and
+    step counts (Logger.cpp:324-391, Hermite4CPU_integration.cpp:102-129)."""
+    from gravidy_amd.checkpoint import read_checkpoint
+    r, v = ic.plummer(1024, 7171)
+    path = tmp_path / "ic.txt"
+    ic.write_ic(path, r, v)
+    _, r_in, v_in, _ = ic.read_ic(path)
+    ref = oracle.integrate(r_in, v_in, softening_e2(1e-4), 0.5)
+    base = [DRIVER, "-i", str(path), "--precision", "exact"]
+    for extra, out in ((["-t", "0.25", "--checkpoint", str(tmp_path / "half.ckpt")], "half"),
+                       (["-t", "0.5", "--restart", str(tmp_path / "half.ckpt"),
+                         "--checkpoint", str(tmp_path / "rest.ckpt")], "rest")):
+        proc = subprocess.run(base + ["-o", str(tmp_path / out)] + extra,
+                              capture_output=True, text=True, timeout=300)
+        assert proc.returncode == 0, proc.stderr
+    c = read_checkpoint(str(tmp_path / "rest.ckpt"))
+    for key in ("r", "f", "t", "dt"):
+        assert c[key].tobytes() == np.ascontiguousarray(ref[key]).tobytes(), key
+    assert c["v"][:, :3].tobytes() == np.ascontiguousarray(ref["v"][:, :3]).tobytes()
+    assert c["interactions"] == ref["interactions"]
+    rows = [ln.split() for ln in open(str(tmp_path / "rest") + ".out.log").read().splitlines()
+            if not ln.startswith("#")]
+    assert [float(x[1]) for x in rows] == [0.375, 0.5]
+    np.testing.assert_allclose([float(x[4]) for x in rows], ref["energy"][3:], rtol=2e-10)
+    # Iter and Nsteps are logged before the output step itself is counted
+    for row, t_out in zip(rows, (0.375, 0.5)):
+        k = int(np.searchsorted(ref["step_time"], t_out))  # index of the output step
+        assert ref["step_time"][k] == t_out
+        assert int(row[2]) == k and int(row[3]) == int(ref["step_nact"][:k].sum())
+
+
+def test_driver_text_resume_matches_oracle(oracle, tmp_path):
+    """-r <run>.info, exact precision (OptionsParser.cpp:122-324,
+    NbodySystem.cpp:106-146, Hermite4CPU_integration.cpp:38-130): a run to
+    t=0.25, then a resumed run to t=0.5 from its .info file and last text
+    snapshot.  The resumed run starts from the snapshot's six-digit values
+    at its "# Time:" and matches the oracle's run started there
+    (h4o_integrate_at): the logged energies, output times, iteration and
+    step counts, and the final snapshot."""
+    r, v = ic.plummer(1024, 8181)
+    path = tmp_path / "ic.txt"
+    ic.write_ic(path, r, v)
+    out = str(path) + ".out"
+    proc = subprocess.run([DRIVER, "-i", str(path), "-t", "0.25", "--precision", "exact"],
+                          capture_output=True, text=True, timeout=300)
+    assert proc.returncode == 0, proc.stderr
+    info = open(out + ".info").read().splitlines()
+    assert info[-1].split()[-1] == "3"
+    _, rs, vs, ts = ic.read_ic(out + ".snapshot_03")
+    assert ts == 0.25
+    ref = oracle.integrate(rs, vs, softening_e2(1e-4), 0.5, itime0=ts)
+
+    proc = subprocess.run([DRIVER, "-r", out + ".info", "-t", "0.5", "--precision", "exact"],
+                          capture_output=True, text=True, timeout=300)
+    assert proc.returncode == 0, proc.stderr
+    assert os.path.exists(out + ".info.old")
+    lines = open(out + ".log").read().splitlines()
+    rows = [ln.split() for ln in lines]
+    # no header: it is printed only at ITIME = 0 (Logger.cpp:349)
+    assert all(not ln.startswith("#") for ln in lines)
+    assert len(rows) == ref["energy"].size
+    np.testing.assert_allclose([float(x[1]) for x in rows], ref["energy_time"], atol=5e-4)
+    np.testing.assert_allclose([float(x[4]) for x in rows], ref["energy"], rtol=2e-10)
+    for row, t_out in zip(rows[1:], ref["energy_time"][1:]):
+        k = int(np.searchsorted(ref["step_time"][1:], t_out)) + 1  # the output step
+        assert ref["step_time"][k] == t_out
+        assert int(row[2]) == k and int(row[3]) == int(ref["step_nact"][:k].sum())
+    last = int(open(out + ".info").read().splitlines()[-1].split()[-1])
+    _, r_end, v_end, t_snap = ic.read_ic(out + f".snapshot_{last:02d}")
+    assert t_snap == 0.5
+    np.testing.assert_allclose(r_end[:, :3], ref["r"][:, :3], rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(v_end[:, :3], ref["v"][:, :3], rtol=1e-6, atol=1e-6)

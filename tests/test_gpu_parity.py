@@ -13,13 +13,21 @@ sequential sum carries an error of that size itself; for those particles
 the bound is 1e-13 * sum_j |a_ij| (resp. |j_ij|), the condition-scaled
 bound of a floating-point sum.
 
+The condition-scaled bound is for the few particles whose pair terms
+cancel: at most FALLBACK_MAX(k) of k checked particles (8, or 0.05 % of
+large samples) may need it; more than that is a wrong force, not
+cancellation.
+
 MIXED (FP32 pair terms): every pair term carries a few FP32 roundings
-(~1e-7 of itself), so the per-particle error is a statistic over
-particles: the root-mean-square of |a_gpu - a_ref| / |a_ref| must be
-<= 1e-7 (the target relative force error) and that of the jerk
-<= 5e-7; median, 99th percentile and maximum are printed.  Its cumulative
-energy error over a whole run must stay within 2x the FP64 path's.  Energies are compared to 1e-13 relative (only the
-summation order differs).
+(~1e-7 of itself), so the error is stated as a statistic over particles
+plus a per-particle cap: the root-mean-square of |a_gpu - a_ref| / |a_ref|
+must be <= 1e-7 (the target relative force error) and that of the jerk
+<= 5e-7, and no particle may exceed MIXED_ACC_MAX = 2e-6 (acceleration)
+or MIXED_JERK_MAX = 2e-5 (jerk), about twice the largest errors measured
+over every particle of the N = 262144 Plummer sphere (1.0e-6 and 1.0e-5).
+Its cumulative energy error over a whole run must stay within 2x the FP64
+path's.  Energies are compared to 1e-13 relative (only the summation order
+differs).
 """
 import numpy as np
 import pytest
@@ -30,6 +38,16 @@ pytestmark = pytest.mark.gpu
 
 E2 = softening_e2(1e-4)
 REL_TOL = 1e-13
+MIXED_ACC_RMS = 1e-7
+MIXED_JERK_RMS = 5e-7
+MIXED_ACC_MAX = 2e-6
+MIXED_JERK_MAX = 2e-5
+
+
+def fallback_max(k):
+    """Largest number of the k checked particles allowed to fall back to the
+    condition-scaled FP64 bound."""
+    return max(8, k // 2000)
 
 
 def bits_equal(a, b):
@@ -73,7 +91,7 @@ def check_forces(f, fo, idx, p, pm, e2, label, tol=REL_TOL):
     over = np.nonzero((da > tol * na) | (dj > tol * nj))[0]
     # The condition-scaled bound is for the few particles whose terms cancel;
     # many particles over the bound are wrong forces, not cancellation.
-    assert over.size <= max(64, idx.size // 100), (
+    assert over.size <= fallback_max(idx.size), (
         f"{label}: {over.size} of {idx.size} particles above {tol:g} norm-wise, "
         f"max rel err acc {ra.max():.3e} jerk {rj.max():.3e}")
     worst = 0.0
@@ -378,29 +396,32 @@ def test_device_loop_matches_host_loop(oracle):
         assert np.array_equal(st, ra["step_time"][5:5 + len(st)])
 
 
-def test_plummer1024_run_fp64_energy_error_matches(oracle):
-    """Fast FP64 kernel over the same run: the cumulative energy error at
-    t_end (CumE, Logger.cpp:328) agrees with the oracle's to 2 significant
-    digits; the schedule agreement is reported."""
+@pytest.mark.parametrize("small_path", [64, 0])
+def test_plummer1024_run_fp64_schedule_and_energy_error_match(oracle, small_path):
+    """Fast FP64 kernel over the same run, with the one-workgroup step for
+    small active sets (the default, 64) and without it (0): the same
+    block-timestep schedule as the oracle (ITIME and nact of every block
+    step), and a cumulative energy error at t_end (CumE, Logger.cpp:328)
+    equal to the oracle's to 2 significant digits."""
     r, v = ic.plummer(1024, 12345)
     ref = oracle.integrate(r, v, E2, 1.0)
     with Hermite4HIP(r, v, E2, precision="fp64") as h:
+        h.set_small_path(small_path)
         run = h.integration(1.0)
-    same = (run["step_time"].shape == ref["step_time"].shape
-            and np.array_equal(run["step_time"], ref["step_time"])
-            and np.array_equal(run["step_nact"], ref["step_nact"]))
+        small = h.stats()["small_steps"]
+    assert (small > 0) == (small_path > 0)
     cum = abs((run["energy"][-1] - run["energy"][0]) / run["energy"][0])
     cum_ref = abs((ref["energy"][-1] - ref["energy"][0]) / ref["energy"][0])
-    print(f"fp64 N=1024: schedule identical={same} steps={len(run['step_time'])}/"
-          f"{len(ref['step_time'])} CumE={cum:.6e} oracle CumE={cum_ref:.6e}")
+    print(f"fp64 N=1024 small_path={small_path}: steps={len(run['step_time'])}/"
+          f"{len(ref['step_time'])} ({small} in one workgroup) CumE={cum:.6e} "
+          f"oracle CumE={cum_ref:.6e}")
+    assert np.array_equal(run["step_time"], ref["step_time"])
+    assert np.array_equal(run["step_nact"], ref["step_nact"])
+    assert run["interactions"] == ref["interactions"]
     assert float(f"{cum:.1e}") == float(f"{cum_ref:.1e}") or abs(cum - cum_ref) <= 0.05 * cum_ref
 
 
 # --- mixed precision ---------------------------------------------------------
-
-MIXED_ACC_RMS = 1e-7
-MIXED_JERK_RMS = 5e-7
-
 
 def check_mixed(f, fo, idx, label):
     ra = np.linalg.norm(f[idx, :3] - fo[idx, :3], axis=1) / np.linalg.norm(fo[idx, :3], axis=1)
@@ -412,7 +433,7 @@ def check_mixed(f, fo, idx, label):
           f"median {np.median(rj):.3e} p99 {np.percentile(rj, 99):.3e} max {rj.max():.3e}")
     assert rms_a <= MIXED_ACC_RMS
     assert rms_j <= MIXED_JERK_RMS
-    assert ra.max() <= 1e-4 and rj.max() <= 1e-3
+    assert ra.max() <= MIXED_ACC_MAX and rj.max() <= MIXED_JERK_MAX
 
 
 @pytest.mark.parametrize("n,nact", [(1024, 1024), (8192, 8192), (65536, 3), (65536, 37),
