@@ -33,6 +33,7 @@ class OptionsParser {
 
     bool file_exists(std::string filename);
     bool check_options();
+    bool check_resume();
     void usage(std::ostream &os) const;
 
   private:

@@ -387,7 +387,14 @@ void h4o_get_energy(uint32_t n, const double *r, const double *v,
  *
  * r, v are updated in place; f, t, dt, a2, a3 receive the final state.
  * Returns the number of block steps taken, or -1 if max_steps was hit
- * first (the recorded prefix is valid). */
+ * first (the recorded prefix is valid).
+ *
+ * h4o_integrate_at starts at ITIME = itime0 instead of 0: the reference's
+ * resumed run (ITIME = ns->snapshot_time, Hermite4CPU_integration.cpp:43),
+ * which re-initialises forces and steps at that time (init_dt sets
+ * t_i = ITIME and leaves ATIME at the smallest dt, so the first block step
+ * of a resumed run, at that ATIME, has no active particle) and counts
+ * output times from output_factor = 1 again. */
 typedef struct {
     double *step_time;      /* [max_steps] */
     uint32_t *step_nact;    /* [max_steps] */
@@ -400,18 +407,18 @@ typedef struct {
     uint64_t nsteps;        /* out: sum of nact */
 } h4o_record;
 
-int64_t h4o_integrate(uint32_t n, double *r, double *v, double *f,
-                      double *t, double *dt, double *a2, double *a3,
-                      double e2, float eta_s, float eta_n,
-                      float integration_time, float interval_time,
-                      double dt_min, double dt_max, h4o_record *rec)
+int64_t h4o_integrate_at(uint32_t n, double *r, double *v, double *f,
+                         double *t, double *dt, double *a2, double *a3,
+                         double e2, float eta_s, float eta_n, double itime0,
+                         float integration_time, float interval_time,
+                         double dt_min, double dt_max, h4o_record *rec)
 {
     double *p = (double *)calloc((size_t)n * 6, sizeof(double));
     float *pm = (float *)calloc(n ? n : 1, sizeof(float));
     double *old = (double *)calloc((size_t)n * 6, sizeof(double));
     uint32_t *move = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
     double atime = 1.0e+10;
-    double itime = 0.0;
+    double itime = itime0;
     unsigned int output_factor = 1;
     int64_t steps = 0;
     double ekin, epot;
@@ -478,6 +485,60 @@ int64_t h4o_integrate(uint32_t n, double *r, double *v, double *f,
     free(old);
     free(move);
     return steps;
+}
+
+/* The body of Hermite4CPU::integration's loop
+ * (Hermite4CPU_integration.cpp:83-99) for `steps` block steps from a given
+ * state: ITIME = *atime; find, save old, predict, update, correct; ATIME =
+ * the next block time.  r, v, f, t, dt, a2, a3 are updated in place and
+ * *atime returns the next block time.  Records ITIME and nact of each step
+ * (rec->max_steps >= steps entries; energies are not computed).  Used to
+ * time the CPU path over the same block steps as the GPU from a state
+ * whose initial forces were computed elsewhere. */
+int64_t h4o_block_steps(uint32_t n, double *r, double *v, double *f,
+                        double *t, double *dt, double *a2, double *a3,
+                        double e2, float eta_n, double *atime, uint64_t steps,
+                        double dt_min, double dt_max, h4o_record *rec)
+{
+    double *p = (double *)calloc((size_t)n * 6, sizeof(double));
+    float *pm = (float *)calloc(n ? n : 1, sizeof(float));
+    double *old = (double *)calloc((size_t)n * 6, sizeof(double));
+    uint32_t *move = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    uint64_t k;
+    rec->n_energy = 0;
+    rec->interactions = 0;
+    rec->nsteps = 0;
+    if (steps > rec->max_steps)
+        steps = rec->max_steps;
+    for (k = 0; k < steps; k++) {
+        double itime = *atime;
+        uint32_t nact = h4o_find_particles_to_move(n, itime, t, dt, move);
+        h4o_save_old_acc_jrk(nact, move, f, old);
+        h4o_predicted_pos_vel(n, itime, t, r, v, f, p, pm);
+        h4o_update_acc_jrk(n, nact, move, p, pm, e2, f);
+        h4o_correction_pos_vel(n, itime, nact, move, p, f, old, a2, a3,
+                               r, v, t, dt, eta_n, dt_min, dt_max);
+        rec->interactions += (uint64_t)nact * n;
+        rec->nsteps += nact;
+        *atime = h4o_next_integration_time(n, t, dt);
+        rec->step_time[k] = itime;
+        rec->step_nact[k] = nact;
+    }
+    free(p);
+    free(pm);
+    free(old);
+    free(move);
+    return (int64_t)steps;
+}
+
+int64_t h4o_integrate(uint32_t n, double *r, double *v, double *f,
+                      double *t, double *dt, double *a2, double *a3,
+                      double e2, float eta_s, float eta_n,
+                      float integration_time, float interval_time,
+                      double dt_min, double dt_max, h4o_record *rec)
+{
+    return h4o_integrate_at(n, r, v, f, t, dt, a2, a3, e2, eta_s, eta_n, 0.0,
+                            integration_time, interval_time, dt_min, dt_max, rec);
 }
 
 /* Number of OpenMP threads the parallel loops above use (1 without

@@ -78,6 +78,12 @@ def load():
     lib.h4o_integrate.argtypes = [u32, _d, _d, _d, _d, _d, _d, _d, dbl, flt, flt,
                                   flt, flt, dbl, dbl, ctypes.POINTER(_Record)]
     lib.h4o_integrate.restype = ctypes.c_int64
+    lib.h4o_integrate_at.argtypes = [u32, _d, _d, _d, _d, _d, _d, _d, dbl, flt, flt, dbl,
+                                     flt, flt, dbl, dbl, ctypes.POINTER(_Record)]
+    lib.h4o_integrate_at.restype = ctypes.c_int64
+    lib.h4o_block_steps.argtypes = [u32, _d, _d, _d, _d, _d, _d, _d, dbl, flt, _d,
+                                    ctypes.c_uint64, dbl, dbl, ctypes.POINTER(_Record)]
+    lib.h4o_block_steps.restype = ctypes.c_int64
     lib.h4o_num_threads.restype = ctypes.c_int
     _lib = lib
     return lib
@@ -223,8 +229,9 @@ class Oracle:
 
     def integrate(self, r, v, e2, t_end, interval=0.125, eta_s=ETA_DEFAULT,
                   eta_n=ETA_DEFAULT, dt_min=D_TIME_MIN, dt_max=D_TIME_MAX,
-                  max_steps=10_000_000, max_energy=4096):
-        """Runs Hermite4CPU::integration (Hermite4CPU_integration.cpp:38-130).
+                  max_steps=10_000_000, max_energy=4096, itime0=0.0):
+        """Runs Hermite4CPU::integration (Hermite4CPU_integration.cpp:38-130)
+        from ITIME = itime0 (0, or the start time of a resumed run).
         Returns a dict with the final state and the recorded schedule."""
         n = r.shape[0]
         r = _c(r, np.float64, (n, 4)).copy()
@@ -242,11 +249,12 @@ class Oracle:
                       _p(e_time, ctypes.c_double), _p(e_val, ctypes.c_double),
                       max_steps, max_energy, 0, 0, 0)
         D = ctypes.c_double
-        steps = self.lib.h4o_integrate(n, _p(r, D), _p(v, D), _p(f, D), _p(t, D), _p(dt, D),
-                                       _p(a2, D), _p(a3, D), float(e2), float(np.float32(eta_s)),
-                                       float(np.float32(eta_n)), float(np.float32(t_end)),
-                                       float(np.float32(interval)), float(dt_min), float(dt_max),
-                                       ctypes.byref(rec))
+        steps = self.lib.h4o_integrate_at(n, _p(r, D), _p(v, D), _p(f, D), _p(t, D), _p(dt, D),
+                                          _p(a2, D), _p(a3, D), float(e2),
+                                          float(np.float32(eta_s)), float(np.float32(eta_n)),
+                                          float(itime0), float(np.float32(t_end)),
+                                          float(np.float32(interval)), float(dt_min),
+                                          float(dt_max), ctypes.byref(rec))
         if steps < 0:
             raise RuntimeError(f"oracle integration exceeded max_steps={max_steps}")
         return {
@@ -255,3 +263,31 @@ class Oracle:
             "energy_time": e_time[:rec.n_energy].copy(), "energy": e_val[:rec.n_energy].copy(),
             "interactions": int(rec.interactions), "nsteps": int(rec.nsteps),
         }
+
+    def block_steps(self, state, e2, atime, steps, eta_n=ETA_DEFAULT, dt_min=D_TIME_MIN,
+                    dt_max=D_TIME_MAX):
+        """`steps` block steps of Hermite4CPU::integration's loop body
+        (Hermite4CPU_integration.cpp:83-99) from state = dict of r, v (n,4),
+        f (n,6), t, dt (n,) and a2, a3 (n,3), updated in place, starting at
+        ITIME = atime.  Returns (next block time, step times, step nacts,
+        interactions)."""
+        n = state["r"].shape[0]
+        for key, shape in (("r", (n, 4)), ("v", (n, 4)), ("f", (n, 6)), ("t", (n,)),
+                           ("dt", (n,)), ("a2", (n, 3)), ("a3", (n, 3))):
+            a = state.get(key)
+            if a is None:
+                state[key] = np.zeros(shape)
+            elif a.shape != shape or a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError(f"{key}: need a C-contiguous float64 array of shape {shape}")
+        step_time = np.zeros(max(1, steps))
+        step_nact = np.zeros(max(1, steps), np.uint32)
+        rec = _Record(_p(step_time, ctypes.c_double), _p(step_nact, ctypes.c_uint32),
+                      None, None, steps, 0, 0, 0, 0)
+        D = ctypes.c_double
+        at = ctypes.c_double(atime)
+        k = self.lib.h4o_block_steps(n, _p(state["r"], D), _p(state["v"], D), _p(state["f"], D),
+                                     _p(state["t"], D), _p(state["dt"], D), _p(state["a2"], D),
+                                     _p(state["a3"], D), float(e2), float(np.float32(eta_n)),
+                                     ctypes.byref(at), int(steps), float(dt_min), float(dt_max),
+                                     ctypes.byref(rec))
+        return at.value, step_time[:k].copy(), step_nact[:k].copy(), int(rec.interactions)

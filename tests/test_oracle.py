@@ -348,3 +348,55 @@ def test_plummer_run_conserves_energy(oracle):
     # block times are multiples of the particles' power-of-two steps
     assert np.all(res["dt"] == 2.0 ** np.round(np.log2(res["dt"])))
     assert np.all(np.mod(res["t"], res["dt"]) == 0.0)
+
+
+def test_resumed_run_is_the_run_shifted_in_time(oracle):
+    """A run started at ITIME = 1 (h4o_integrate_at: the reference's resumed
+    run, Hermite4CPU_integration.cpp:43-54) is the run started at 0 shifted
+    by 1 in time: 1 is a multiple of twice every block step, so the block
+    quantisation (normalize_dt's doubling test) and every step length
+    dt = ITIME - t_i are the same, and the final state is bit-identical.
+    Its first block step is at ATIME = the smallest initial dt (init_dt
+    leaves ATIME there), before the particles' times, with no active
+    particle; outputs then follow the reference's counter, which restarts
+    at interval * 1 and so fires at the first steps after the start."""
+    from gravidy_amd import ic
+    r, v = ic.plummer(64, 3)
+    e2 = softening_e2(1e-4)
+    ref = oracle.integrate(r, v, e2, 0.125)
+    res = oracle.integrate(r, v, e2, 1.125, itime0=1.0)
+    assert res["step_nact"][0] == 0 and res["step_time"][0] < 1.0
+    assert res["step_time"][0] == ref["step_time"][0]  # the smallest dt, from time 0 and 1 alike
+    assert np.array_equal(res["step_time"][1:], ref["step_time"] + 1.0)
+    assert np.array_equal(res["step_nact"][1:], ref["step_nact"])
+    for key in ("r", "v", "f", "dt"):
+        assert res[key].tobytes() == ref[key].tobytes(), key
+    assert np.array_equal(res["t"], ref["t"] + 1.0)
+    assert res["energy_time"][0] == 1.0
+    assert res["energy"][0] == ref["energy"][0]
+    # outputs at the first steps with ITIME >= 0.125 k, k = 1, 2, ...
+    k = np.arange(1, res["energy_time"].size)
+    assert np.all(res["energy_time"][1:] >= 0.125 * k)
+    assert res["energy_time"][-1] == 1.125
+
+
+def test_block_steps_continue_an_integration(oracle):
+    """h4o_block_steps from the state after k steps of a run continues it:
+    the next steps, their schedule and the final state are the run's own
+    (Hermite4CPU_integration.cpp:83-99 is the same loop body)."""
+    from gravidy_amd import ic
+    r, v = ic.plummer(128, 8)
+    e2 = softening_e2(1e-4)
+    ref = oracle.integrate(r, v, e2, 2.0 ** -5)
+    total = ref["step_time"].size
+    part = oracle.integrate(r, v, e2, 2.0 ** -7)
+    k = part["step_time"].size
+    assert 0 < k < total
+    state = {key: part[key].copy() for key in ("r", "v", "f", "t", "dt", "a2", "a3")}
+    atime = oracle.next_integration_time(part["t"], part["dt"])
+    atime, st, sn, inter = oracle.block_steps(state, e2, atime, total - k)
+    assert np.array_equal(st, ref["step_time"][k:]) and np.array_equal(sn, ref["step_nact"][k:])
+    assert inter == int(ref["step_nact"][k:].astype(np.int64).sum()) * 128
+    for key in ("r", "v", "f", "t", "dt"):
+        assert state[key].tobytes() == ref[key].tobytes(), key
+    assert atime == oracle.next_integration_time(ref["t"], ref["dt"])
