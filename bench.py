@@ -12,8 +12,13 @@ Also reported: the update kernel's fraction of the FP64 vector peak
 (60 flops per pair, HIP events on the context's stream), the realistic
 block-timestep schedule rate (first block steps from the same IC with the
 reference's dt bounds 2^-23..2^-3), an active-set sweep of the update, the
-mixed-precision path, and the CPU oracle timed on the host cores over a
-bounded sample of the same update.
+mixed-precision path, the CPU oracle timed on the host cores over a
+bounded sample of the same update, whole runs (N = 1024 to t = 1 beside
+the oracle's run, N = 65536 to t = 0.5), the ensemble of 64 clusters to
+t = 1 (the oracle over the same window for its smallest clusters), and the
+N = 262144 and N = 1048577 (Salpeter + central mass) configurations: the
+same first block steps on the oracle and on the GPU (FP64 and mixed), and
+a GPU time window of each.
 """
 import argparse
 import json
@@ -129,15 +134,7 @@ def cpu_baseline(r, v, e2, sample_nact, seed=7):
     t0 = time.perf_counter()
     o.update_acc_jrk(move, p, pm, e2, f)
     el = time.perf_counter() - t0
-    cpu = "unknown"
-    try:
-        with open("/proc/cpuinfo") as fh:
-            for line in fh:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
+    cpu = cpu_model()
     return {"value": sample_nact * n / el, "unit": "pair interactions/s",
             "cores": o.num_threads(), "kind": "port",
             "sample": f"Hermite4CPU::update_acc_jrk restated (oracle, -O3 -fopenmp "
@@ -183,6 +180,107 @@ def run_leg(e2, n, seed, t_end, precision, cpu=False, graphs=True):
                                "block_steps": int(len(ref["step_time"])),
                                "same_schedule": bool(np.array_equal(ref["step_time"],
                                                                     run["step_time"]))}
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def same_steps_leg(name, r, v, e2, dt_min, cpu_budget_s, t_window, interval):
+    """One configuration over the same block steps on the CPU and the GPU,
+    plus a longer GPU time window.
+
+    The initial forces and steps come from the FP64 kernel (h4h_init_acc_jrk,
+    h4h_init_dt); from that common state the oracle (Hermite4CPU's loop body,
+    all host cores) runs block steps in chunks until cpu_budget_s is spent,
+    K steps in all, and the GPU then runs exactly those K steps (device
+    loop, FP64 and mixed, timed after an untimed pass over the same steps).
+    Rates are sum(nact * N) / time over the same steps; same_schedule says
+    whether the GPU's block times and active counts equal the oracle's.
+    Then each precision integrates the configuration from t = 0 to t_window
+    on its own (energies at t = 0, interval, ...), for a whole-run rate and
+    the cumulative energy error."""
+    from gravidy_amd import Hermite4HIP
+    from oracle import Oracle
+    n = r.shape[0]
+    out = {"workload": name, "n": n, "dt_min": dt_min}
+    with Hermite4HIP(r, v, e2, precision="fp64", dt_min=dt_min) as h:
+        t0 = time.perf_counter()
+        h.init_acc_jrk()
+        atime0 = h.init_dt()
+        s0 = h.download()
+        out["init_seconds"] = time.perf_counter() - t0
+    start = {k: np.ascontiguousarray(s0[k]) for k in ("r", "v", "f", "t", "dt")}
+
+    o = Oracle()
+    st = {k: start[k].copy() for k in start}
+    st["a2"] = np.zeros((n, 3))
+    st["a3"] = np.zeros((n, 3))
+    times, nacts, cpu_pairs = [], [], 0
+    at = atime0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < cpu_budget_s and len(times) < 100000:
+        at, stt, snn, inter = o.block_steps(st, e2, at, 8, dt_min=dt_min)
+        times.append(stt)
+        nacts.append(snn)
+        cpu_pairs += inter
+    cel = time.perf_counter() - t0
+    times, nacts = np.concatenate(times), np.concatenate(nacts)
+    K = int(times.size)
+    log(f"[bench] {name}: oracle {K} block steps in {cel:.2f} s on {o.num_threads()} threads")
+    out["block_steps"] = K
+    out["pairs"] = int(cpu_pairs)
+    out["t_reached"] = float(times[-1])
+    out["mean_nact"] = float(nacts.mean())
+    out["median_nact"] = float(np.median(nacts))
+    out["cpu_baseline"] = {"value": cpu_pairs / cel, "unit": "pair interactions/s",
+                           "seconds": cel, "cores": o.num_threads(), "kind": "port",
+                           "sample": f"oracle (Hermite4CPU loop body restated) over the first {K} "
+                                     f"block steps from the FP64 initial forces, "
+                                     f"{o.num_threads()} threads of {cpu_model()}"}
+    for precision in ("fp64", "mixed"):
+        with Hermite4HIP(r, v, e2, precision=precision, dt_min=dt_min) as h:
+            h.upload(start["r"], start["v"], start["f"], start["t"], start["dt"])
+            h.run(0.0, atime0, float("inf"), float("inf"), K, record=False)  # graph capture
+            h.upload(start["r"], start["v"], start["f"], start["t"], start["dt"])
+            h.synchronize()
+            t0 = time.perf_counter()
+            info, gst, gsn = h.run(0.0, atime0, float("inf"), float("inf"), K)
+            el = time.perf_counter() - t0
+        out[precision] = {"seconds": el, "pairs_per_s": info["interactions"] / el,
+                          "us_per_step": el / K * 1e6,
+                          "speedup_vs_cpu": (info["interactions"] / el) / (cpu_pairs / cel),
+                          "same_schedule": bool(np.array_equal(gst, times)
+                                                and np.array_equal(gsn, nacts))}
+        log(f"[bench] {name} {precision}: {K} block steps in {el * 1e3:.1f} ms, "
+            f"{info['interactions'] / el:.3e} pairs/s, same schedule "
+            f"{out[precision]['same_schedule']}")
+    out["mixed"]["speedup_vs_fp64"] = out["mixed"]["pairs_per_s"] / out["fp64"]["pairs_per_s"]
+
+    win = {"t_end": t_window, "interval": interval}
+    for precision in ("fp64", "mixed"):
+        with Hermite4HIP(r, v, e2, precision=precision, dt_min=dt_min) as h:
+            t0 = time.perf_counter()
+            run = h.integration(t_window, interval=interval)
+            el = time.perf_counter() - t0
+        e = run["energy"]
+        win[precision] = {"seconds": el, "block_steps": int(len(run["step_time"])),
+                          "interactions": int(run["interactions"]),
+                          "pairs_per_s": run["interactions"] / el,
+                          "mean_nact": float(run["step_nact"].mean()),
+                          "cum_energy_error": float(abs((e[-1] - e[0]) / e[0]))}
+        log(f"[bench] {name} {precision} to t={t_window}: {len(run['step_time'])} block steps "
+            f"in {el:.2f} s, {run['interactions'] / el:.3e} pairs/s")
+    win["mixed_speedup"] = win["fp64"]["seconds"] / win["mixed"]["seconds"]
+    out["window"] = win
     return out
 
 
@@ -306,21 +404,33 @@ def ensemble_leg(e2, precision, count=64, seed=64, t_end=1.0, cpu_clusters=4, gr
            "cum_energy_error_median": float(np.median(cum)), "cum_energy_error_max": float(cum.max()),
            "launch": choice, "tune_seconds": tune_s}
     if cpu_clusters:
+        # The oracle over the same time window for the smallest clusters,
+        # each cluster's run compared with the GPU's: block steps,
+        # interactions and the cumulative energy error to 2 digits.
         from oracle import Oracle
         o = Oracle()
         order = np.argsort(sizes)[:cpu_clusters]
         t0 = time.perf_counter()
         pairs = 0
+        checks = []
         for k in order:
             r, v = systems[k]
-            ref = o.integrate(r, v, e2, 0.125)
+            ref = o.integrate(r, v, e2, t_end)
             pairs += ref["interactions"]
+            eo = ref["energy"]
+            cum_ref = abs((eo[-1] - eo[0]) / eo[0])
+            checks.append({"cluster": int(k), "n": int(sizes[k]),
+                           "steps": int(res["steps"][k]), "steps_cpu": int(len(ref["step_time"])),
+                           "interactions_equal": int(res["interactions"][k]) == ref["interactions"],
+                           "cum_energy_error": float(cum[k]), "cum_energy_error_cpu": cum_ref,
+                           "cum_energy_error_2_digits": f"{cum[k]:.1e}" == f"{cum_ref:.1e}"})
         cel = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": pairs / cel, "unit": "pair interactions/s",
                                "cores": o.num_threads(), "kind": "port",
                                "sample": f"oracle integration of the {cpu_clusters} smallest "
                                          f"clusters (n={sorted(int(sizes[k]) for k in order)}) "
-                                         f"to t=0.125, {cel:.2f} s"}
+                                         f"to t={t_end}, the GPU's window, {cel:.2f} s",
+                               "clusters": checks}
     return out
 
 
@@ -350,6 +460,11 @@ def main():
     ap.add_argument("--no-runs", action="store_true",
                     help="skip the whole-run legs (N=1024 to t=1 with the CPU oracle beside it, "
                          "N=65536 to t=0.5)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the N=262144 and N=1048577 configuration legs (same block steps "
+                         "on CPU and GPU, FP64 and mixed, and a GPU time window)")
+    ap.add_argument("--cpu-budget", type=float, default=10.0,
+                    help="seconds of oracle block steps per configuration leg")
     ap.add_argument("--salpeter-bh-run", type=float, default=0.0, metavar="T_END",
                     help="only run the N=1048577 Salpeter + central-mass configuration to T_END "
                          "in mixed and FP64 (whole runs, minutes)")
@@ -452,6 +567,20 @@ def main():
                                             cpu=args.cpu_sample > 0, graphs=g),
                 "plummer_n65536_t0.5": run_leg(e2, 65536, 65536, 0.5, args.precision, graphs=g)}
 
+    configs = None
+    if not args.no_configs:
+        configs = {}
+        log("[bench] configuration legs: Plummer N=262144 (to t=1/16), "
+            "Salpeter + central mass N=1048577 (to t=2^-8)")
+        configs["plummer_n262144"] = same_steps_leg(
+            "plummer_n262144_eps1e-4", r, v, e2, 2.0 ** -23, args.cpu_budget, 0.0625, 0.03125)
+        from gravidy_amd import ic as _ic
+        rb, vb = _ic.plummer_salpeter_bh(1048576, 1048576)
+        configs["salpeter_bh_n1048577"] = same_steps_leg(
+            "salpeter_plummer_bh_n1048577_eps1e-5_dtmin2^-30", rb, vb, softening_e2(1e-5),
+            2.0 ** -30, args.cpu_budget, 2.0 ** -8, 2.0 ** -9)
+        del rb, vb
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"[bench] CPU oracle baseline, nact={args.cpu_sample}")
@@ -484,6 +613,7 @@ def main():
         "block_schedule": sched,
         "ensemble": ens,
         "runs": runs,
+        "configs": configs,
         "nact_sweep": sweep,
         "host": platform.node(),
     }
