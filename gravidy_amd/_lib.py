@@ -35,8 +35,12 @@ EXPORTS = (
     "h4h_ensemble_sizes", "h4h_ensemble_init", "h4h_ensemble_run", "h4h_ensemble_energy",
     "h4h_set_tuning", "h4h_autotune", "h4h_set_graphs", "h4h_set_small_path",
     "h4h_tune_launch",
-    "h4h_set_launch_choice",
+    "h4h_set_launch_choice", "h4h_update_acc_jrk_static",
 )
+
+# Limits of h4h_update_acc_jrk_static (include/h4hip.h).
+H4H_STATIC_MAX_N = 65536
+H4H_STATIC_SCRATCH_BYTES = 58130688
 
 
 class H4HStats(ctypes.Structure):
@@ -151,6 +155,8 @@ def load():
         "h4h_set_small_path": ([P, u32], i32),
         "h4h_tune_launch": ([P, u32, ctypes.POINTER(H4HLaunchChoice)], i32),
         "h4h_set_launch_choice": ([P, ctypes.POINTER(H4HLaunchChoice)], i32),
+        # device pointers as integers (e.g. a torch tensor's data_ptr())
+        "h4h_update_acc_jrk_static": ([P, P, u32, P, u32, dbl, i32, P, P], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

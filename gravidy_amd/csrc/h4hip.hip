@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2139,11 +2140,139 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
     return H4H_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Context-free update with a static device buffer (h4h_update_acc_jrk_static)
+// ---------------------------------------------------------------------------
+//
+// Everything the update and finish kernels need beyond the caller's inputs
+// and outputs lives in `scratch`, one file-scope device array per GPU (the
+// loader gives each device its own copy): tables of one cluster, the
+// active list (range-checked copy), the predictor records and the forces in
+// the kernels' layouts, and the j-split partial sums.  Offsets in doubles,
+// each a multiple of 32 (256 bytes).
+//
+// Partial sums: for n <= STATIC_MAX_N with the default launch (grid <= 2048
+// workgroups, queue target 16 items per workgroup, at most 256 i slots per
+// item), a plan with S > 1 chunks has S <= max(s_fill, s_tiles) where
+// s_fill = ceil(grid / n_it) and s_tiles = floor(n_jt / CHUNK_TILES) <= 16:
+// S n_it iw <= (grid + n_it - 1) iw <= grid BS + n + BS - 1 in the first
+// case and S rows <= 16 (n + BS - 1) in the second, so 6 S rows <=
+// ST_PART doubles.  The call also checks its own plan against ST_PART.
+constexpr uint32_t STATIC_MAX_N = 65536;
+static_assert(STATIC_MAX_N == H4H_STATIC_MAX_N, "header and library agree");
+constexpr size_t st_round32(size_t x) { return (x + 31) / 32 * 32; }
+constexpr size_t ST_PART_ROWS_FILL = (size_t)DEFAULT_GRID * BS + STATIC_MAX_N + BS - 1;
+constexpr size_t ST_PART_ROWS_TILES =
+    (size_t)(STATIC_MAX_N / JT / CHUNK_TILES) * (STATIC_MAX_N + BS - 1);
+constexpr size_t ST_PART =
+    st_round32(6 * (ST_PART_ROWS_FILL > ST_PART_ROWS_TILES ? ST_PART_ROWS_FILL : ST_PART_ROWS_TILES));
+constexpr size_t ST_CTL = 0, ST_CS = 32, ST_PLAN = 64, ST_PRE = 80, ST_TAB = 96;  // header
+constexpr size_t ST_MOVE = 128;                                   // u32 x STATIC_MAX_N
+constexpr size_t ST_PP = ST_MOVE + STATIC_MAX_N / 2;              // double4 x STATIC_MAX_N
+constexpr size_t ST_PV = ST_PP + 4 * (size_t)STATIC_MAX_N;
+constexpr size_t ST_F = ST_PV + 4 * (size_t)STATIC_MAX_N;         // 6 columns
+constexpr size_t ST_PARTIAL = ST_F + 6 * (size_t)STATIC_MAX_N;
+constexpr size_t ST_TOTAL = ST_PARTIAL + ST_PART;
+static_assert(ST_TOTAL * sizeof(double) == H4H_STATIC_SCRATCH_BYTES, "header states the size");
+static_assert(sizeof(RunCtl) <= 32 * sizeof(double) && sizeof(ClState) <= 32 * sizeof(double) &&
+                  sizeof(ClPlan) <= 16 * sizeof(double) &&
+                  6 * sizeof(unsigned long long) <= 16 * sizeof(double),
+              "header block of the static buffer");
+static_assert(ST_MOVE % 32 == 0 && ST_PP % 32 == 0 && ST_PV % 32 == 0 && ST_F % 32 == 0 &&
+                  ST_PARTIAL % 32 == 0,
+              "256-byte sections");
+
+// ST_TAB holds off, cnt, hoff of the one cluster and the bad-index flag.
+enum : uint32_t { TAB_OFF = 0, TAB_CNT = 1, TAB_HOFF = 2, TAB_BAD = 3 };
+
+// Tables of one cluster of n particles with nact active, and the work-queue
+// counters at zero (a fill kernel, so the call can be captured in a graph).
+__global__ void k_static_reset(Ens e, uint32_t *tab, uint32_t n, uint32_t nact) {
+    ClState c{};
+    c.nact = nact;
+    e.cs[0] = c;
+    RunCtl r{};
+    *e.ctl = r;
+    tab[TAB_OFF] = 0;
+    tab[TAB_CNT] = n;
+    tab[TAB_HOFF] = 0;
+    tab[TAB_BAD] = 0;
+}
+
+// Predictor records of p6 (n x 6: r, v) and pm (float masses) in the tile
+// layout, zero-mass padding up to n_pad; the active list copied with every
+// index >= n replaced by 0 and flagged in tab[TAB_BAD].
+__global__ __launch_bounds__(BS) void k_static_pack(const double *p6, const float *pm, uint32_t n,
+                                                   uint32_t n_pad, const uint32_t *move_in,
+                                                   uint32_t nact, uint32_t *move, Soa s,
+                                                   uint32_t *tab) {
+    const uint32_t i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) {
+        const float mf = pm[i];
+        const double *p = p6 + (size_t)i * 6;
+        s.pp[i] = make_double4(p[0], p[1], p[2], (double)mf);
+        s.pv[i] = make_double4(p[3], p[4], p[5], mass_bits(mf));
+    } else if (i < n_pad) {
+        s.pp[i] = make_double4(PAD_POS, PAD_POS, PAD_POS, 0.0);
+        s.pv[i] = make_double4(0.0, 0.0, 0.0, mass_bits(0.0f));
+    }
+    if (i < nact) {
+        uint32_t j = move_in[i];
+        if (j >= n) {
+            atomicOr(&tab[TAB_BAD], 1u);
+            j = 0;
+        }
+        move[i] = j;
+    }
+}
+
+// Rows move_in[k] of f6 (n x 6) from the force columns; every row NaN if
+// the list held an index >= n (those indices are skipped).
+__global__ __launch_bounds__(BS) void k_static_out(Soa s, const uint32_t *move_in, uint32_t n,
+                                                  uint32_t nact, const uint32_t *tab, double *f6) {
+    const uint32_t k = blockIdx.x * BS + threadIdx.x;
+    if (k >= nact) return;
+    const uint32_t i = move_in[k];
+    if (i >= n) return;
+    const bool bad = tab[TAB_BAD] != 0;
+    double *row = f6 + (size_t)i * 6;
+    for (int q = 0; q < 6; ++q) row[q] = bad ? (double)NAN : s.f[q][i];
+}
+
+}  // namespace
+
+// The static buffer (one per GPU).  Host code uses the device address
+// from hipGetSymbolAddress, never this name.
+static __device__ __align__(256) double scratch[ST_TOTAL];
+
+namespace {
+
+std::mutex g_static_mu;
+constexpr int STATIC_MAX_DEVICES = 256;
+double *g_static_ptr[STATIC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the first call there).
+int static_scratch(double **out) {
+    int dev = 0;
+    H4H_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= STATIC_MAX_DEVICES) return fail(H4H_ENODEV, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (!g_static_ptr[dev]) {
+        void *p = nullptr;
+        H4H_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(H4H_EHIP, "hipGetSymbolAddress returned null");
+        g_static_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_static_ptr[dev];
+    return H4H_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
-int h4h_abi_version(void) { return 106; }
+int h4h_abi_version(void) { return 107; }
 
 const char *h4h_last_error(void) { return g_err.c_str(); }
 
@@ -2869,6 +2998,77 @@ int h4h_synchronize(h4h_ctx *ctx) {
     int rc = check_ctx(ctx, false);
     if (rc) return rc;
     H4H_HIP(hipStreamSynchronize(ctx->stream));
+    return H4H_OK;
+}
+
+int h4h_update_acc_jrk_static(const double *d_p6, const float *d_pm, uint32_t n,
+                              const uint32_t *d_move, uint32_t nact, double e2, int precision,
+                              double *d_f6, void *stream) {
+    if (n > STATIC_MAX_N) return fail(H4H_EINVAL, "n exceeds H4H_STATIC_MAX_N (65536)");
+    if (nact > n) return fail(H4H_EINVAL, "nact > n");
+    if (!(e2 >= 0.0) || !std::isfinite(e2)) return fail(H4H_EINVAL, "e2 must be finite and >= 0");
+    if (!valid_precision(precision)) return fail(H4H_EINVAL, "unknown precision");
+    if (nact && (!d_p6 || !d_pm || !d_move || !d_f6))
+        return fail(H4H_EINVAL, "null device pointer");
+    double *base = nullptr;
+    int rc = static_scratch(&base);
+    if (rc) return rc;
+    if (nact == 0) return H4H_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t n_pad = (n + JT - 1) / JT * JT;
+
+    // The plan of a default single-system context of n particles.
+    const uint32_t grid = std::min(DEFAULT_GRID, max_items(n, DEFAULT_GRID, DEFAULT_MAX_SPLIT,
+                                                           DEFAULT_IW_FILL));
+    if (precision != H4H_FP64_EXACT) {
+        const Decomp d = decompose(nact, pick_iw(nact, DEFAULT_IW_FILL), n_pad / JT, grid,
+                                   DEFAULT_MAX_SPLIT);
+        if (d.S > 1 && 6ull * d.S * d.rows > ST_PART)
+            return fail(H4H_EINVAL, "work plan exceeds the static buffer");
+    }
+    Soa s{};
+    s.pp = reinterpret_cast<double4 *>(base + ST_PP);
+    s.pv = reinterpret_cast<double4 *>(base + ST_PV);
+    for (int q = 0; q < 6; ++q) s.f[q] = base + ST_F + (size_t)q * STATIC_MAX_N;
+    uint32_t *tab = reinterpret_cast<uint32_t *>(base + ST_TAB);
+    uint32_t *move = reinterpret_cast<uint32_t *>(base + ST_MOVE);
+    unsigned long long *pre = reinterpret_cast<unsigned long long *>(base + ST_PRE);
+    double *partial = base + ST_PARTIAL;
+    Ens e{};
+    e.K = 1;
+    e.nblk = n_pad / JT;
+    e.off = tab + TAB_OFF;
+    e.cnt = tab + TAB_CNT;
+    e.hoff = tab + TAB_HOFF;
+    e.cs = reinterpret_cast<ClState *>(base + ST_CS);
+    e.plan = reinterpret_cast<ClPlan *>(base + ST_PLAN);
+    e.items_pre = pre;
+    e.fin_pre = pre + 2;
+    e.part_pre = pre + 4;
+    e.ctl = reinterpret_cast<RunCtl *>(base + ST_CTL);
+    e.grid = grid;
+    e.max_split = DEFAULT_MAX_SPLIT;
+    e.iw_fill = DEFAULT_IW_FILL;
+    e.tuned = 0;
+
+    hipLaunchKernelGGL(k_static_reset, dim3(1), dim3(1), 0, st, e, tab, n, nact);
+    H4H_LAUNCHED();
+    hipLaunchKernelGGL(k_static_pack, dim3(n_pad / BS), dim3(BS), 0, st, d_p6, d_pm, n, n_pad,
+                       d_move, nact, move, s, tab);
+    H4H_LAUNCHED();
+    if (precision == H4H_FP64_EXACT) {
+        hipLaunchKernelGGL(k_update_exact, dim3(grid), dim3(BS), 0, st, s, e, move, e2);
+    } else {
+        auto *kern = precision == H4H_MIXED ? k_update_mixed : k_update_fp64;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(BS), 0, st, s, e, move, partial, e2);
+    }
+    H4H_LAUNCHED();
+    hipLaunchKernelGGL(k_finish, dim3(std::min<uint32_t>(1024, std::max<uint32_t>(64, e.nblk))),
+                       dim3(BS), 0, st, s, e, move, partial, FIN_WRITE_F, 0.0f, 0.0, 0.0);
+    H4H_LAUNCHED();
+    hipLaunchKernelGGL(k_static_out, dim3((nact + BS - 1) / BS), dim3(BS), 0, st, s, d_move, n,
+                       nact, tab, d_f6);
+    H4H_LAUNCHED();
     return H4H_OK;
 }
 

@@ -325,6 +325,40 @@ int h4h_get_stats(h4h_ctx *ctx, h4h_stats *out, int reset);
 /* Blocks until all work queued on the context's stream is done. */
 int h4h_synchronize(h4h_ctx *ctx);
 
+/* ---- Update without a context and without allocation -----------------
+ * No counterpart in the reference.  For callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The call makes no hipMalloc, hipFree or
+ * other allocating call; the work plan, work-queue counters, j-split
+ * partial sums and the predictor and forces in the kernels' layouts live in
+ * one static device array of H4H_STATIC_SCRATCH_BYTES that every GPU the
+ * library runs on holds until the process ends.  Every call on a GPU
+ * overwrites that array, so calls that overlap on one GPU (several host
+ * threads, or streams that run at the same time) race. */
+#define H4H_STATIC_MAX_N          65536u     /* largest n of the call      */
+#define H4H_STATIC_SCRATCH_BYTES  58130688u  /* the static array, per GPU  */
+
+/* h4h_update_acc_jrk on device arrays of the current device, with no
+ * context: acceleration and jerk of the nact particles d_move[0..nact)
+ * over all n particles, j != i.
+ *   d_p6   : n x 6 double, predicted r[3], v[3] (Predictor, common.hpp:213-217)
+ *   d_pm   : n float, Predictor::m
+ *   d_move : nact uint32 indices < n
+ *   d_f6   : n x 6 double (Forces, common.hpp:228-231): rows d_move[k]
+ *            receive a[3], a1[3]; other rows are left as they are
+ * e2 and precision as h4h_create; n <= H4H_STATIC_MAX_N, nact <= n (larger
+ * sizes are refused with H4H_EINVAL).  The work is queued on `stream` (a
+ * hipStream_t; NULL is the null stream) and not waited for.  The results
+ * are bit for bit those of h4h_upload_predictor + h4h_set_active +
+ * h4h_update_acc_jrk on a context of n particles with the default launch.
+ * An index >= n in d_move is caught on the device: every row d_move[k]
+ * < n of d_f6 is then NaN.  The first call on a device looks up the static
+ * array's address there; a call with nact = 0 does only that, so a caller
+ * that captures graphs can make it before capturing. */
+int h4h_update_acc_jrk_static(const double *d_p6, const float *d_pm, uint32_t n,
+                              const uint32_t *d_move, uint32_t nact, double e2,
+                              int precision, double *d_f6, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

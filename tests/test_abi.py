@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_error_text():
     lib = _lib.load()
-    assert lib.h4h_abi_version() == 106
+    assert lib.h4h_abi_version() == 107
     assert isinstance(lib.h4h_last_error(), bytes)
 
 
@@ -67,3 +67,16 @@ def test_null_context_is_an_error():
     assert lib.h4h_set_small_path(None, 64) == _lib.H4H_EINVAL
     assert lib.h4h_ensemble_energy(None, None, None) == _lib.H4H_EINVAL
     assert lib.h4h_create(None, 16, 1e-8, 0.01, 2.0 ** -23, 0.125, 0) == _lib.H4H_EINVAL
+
+
+def test_static_update_rejects_bad_sizes():
+    """h4h_update_acc_jrk_static refuses, before any device call, sizes
+    beyond its static buffer and missing pointers."""
+    lib = _lib.load()
+    f = lib.h4h_update_acc_jrk_static
+    assert f(None, None, _lib.H4H_STATIC_MAX_N + 1, None, 1, 1e-8, 0, None, None) == _lib.H4H_EINVAL
+    assert b"65536" in lib.h4h_last_error()
+    assert f(None, None, 16, None, 17, 1e-8, 0, None, None) == _lib.H4H_EINVAL
+    assert f(None, None, 16, None, 4, 1e-8, 0, None, None) == _lib.H4H_EINVAL
+    assert f(None, None, 16, None, 4, -1.0, 0, None, None) == _lib.H4H_EINVAL
+    assert f(None, None, 16, None, 4, 1e-8, 9, None, None) == _lib.H4H_EINVAL
