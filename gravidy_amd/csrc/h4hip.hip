@@ -397,7 +397,12 @@ __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, dou
                 total += n;
             }
             wave_off[BS / 64] = total ? atomicAdd(&cs->nact, total) : 0;
-            if (total && (flags & PS_DIRTY)) e.dirty[atomicAdd(&e.ctl->ndirty, 1u)] = b;
+            if (total && (flags & PS_DIRTY)) {
+                const uint32_t d = atomicAdd(&e.ctl->ndirty, 1u);
+                e.dirty[3 * d] = b;
+                e.dirty[3 * d + 1] = wave_off[BS / 64];
+                e.dirty[3 * d + 2] = total;
+            }
         }
         __syncthreads();
         if (act) {
@@ -868,13 +873,55 @@ __global__ __launch_bounds__(BS) void k_correct(Soa s, Ens e, const uint32_t *mo
 
 enum : uint32_t { FIN_WRITE_F = 0, FIN_CORRECT = 1 };
 
+// Sum of active particle k's S j-chunk partials (plan pl, partials of its
+// cluster at part) on the group of L = pl.lanes consecutive lanes of one
+// wave that share it: lane sl adds chunks sl, sl + L, ... in that order,
+// then a fixed xor butterfly over the group, so the order is fixed for a
+// given plan and results are deterministic.  With one chunk the update
+// stored the forces in f[i].  Every lane of the group gets the sums.
+__device__ __forceinline__ void chunk_sums(const Soa &s, const ClPlan &pl, const double *part,
+                                           uint32_t k, uint32_t i, bool valid, double f[6]) {
+    const uint32_t L = pl.lanes;
+    const uint32_t sl = threadIdx.x % L;
+    if (pl.S == 1) {
+        for (int q = 0; q < 6; ++q) f[q] = valid ? s.f[q][i] : 0.0;
+        return;
+    }
+    // The loads of a round of U chunks of all six components are issued
+    // together (a few memory latencies per round instead of one per chunk;
+    // S <= 1024 takes two rounds at L = 64).
+    constexpr uint32_t U = 8;
+    const double *pk = part + k;
+    const size_t qs = (size_t)pl.S * pl.rows;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (valid) {
+        for (uint32_t s0 = sl; s0 < pl.S; s0 += U * L) {
+            double v[6][U];
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t sc = s0 + u * L;
+                    v[q][u] = sc < pl.S ? pk[q * qs + (size_t)sc * pl.rows] : 0.0;
+                }
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u)
+                    if (s0 + u * L < pl.S) acc[q] += v[q][u];
+        }
+    }
+    for (int q = 0; q < 6; ++q) {
+        for (uint32_t o = 1; o < L; o <<= 1) acc[q] += __shfl_xor(acc[q], o);
+        f[q] = acc[q];
+    }
+}
+
 // Finish pass of an update: adds the j-chunk partials of each active
-// particle and either stores the forces (FIN_WRITE_F) or applies the
-// corrector at the cluster's block time ATIME (FIN_CORRECT).  A particle's
-// S chunk sums are spread over L = pl.lanes lanes (chunks sl, sl + L, ...
-// summed in order by lane sl, then a fixed xor butterfly), so a workgroup
-// finishes BS/L particles with no barrier; the order is fixed for a given
-// plan, so results are deterministic.
+// particle (chunk_sums) and either stores the forces (FIN_WRITE_F) or
+// applies the corrector at the cluster's block time ATIME (FIN_CORRECT).
+// A particle's chunk sums are spread over L = pl.lanes lanes, so a
+// workgroup finishes BS/L particles with no barrier.
 __global__ __launch_bounds__(BS) void k_finish(Soa s, Ens e, const uint32_t *move,
                                               const double *partial, uint32_t mode, float eta,
                                               double dt_min, double dt_max) {
@@ -886,44 +933,11 @@ __global__ __launch_bounds__(BS) void k_finish(Soa s, Ens e, const uint32_t *mov
         const uint32_t lu = (uint32_t)(u - e.fin_pre[c]);
         const uint32_t nact = e.cs[c].nact;
         const uint32_t k = lu * (BS / L) + threadIdx.x / L;
-        const uint32_t sl = threadIdx.x % L;
         const bool valid = k < nact;
         const uint32_t i = valid ? move[e.off[c] + k] : 0u;
         double f[6];
-        if (pl.S == 1) {
-            for (int q = 0; q < 6; ++q) f[q] = valid ? s.f[q][i] : 0.0;
-        } else {
-            // Lane sl adds chunks sl, sl + L, ... in that order.  The loads
-            // of a round of U chunks of all six components are issued
-            // together (a few memory latencies per round instead of one per
-            // chunk; S <= 1024 takes two rounds at L = 64).
-            constexpr uint32_t U = 8;
-            const double *part = partial + e.part_pre[c] + k;
-            const size_t qs = (size_t)pl.S * pl.rows;
-            double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            if (valid) {
-                for (uint32_t s0 = sl; s0 < pl.S; s0 += U * L) {
-                    double v[6][U];
-#pragma unroll
-                    for (int q = 0; q < 6; ++q)
-#pragma unroll
-                        for (uint32_t u = 0; u < U; ++u) {
-                            const uint32_t sc = s0 + u * L;
-                            v[q][u] = sc < pl.S ? part[q * qs + (size_t)sc * pl.rows] : 0.0;
-                        }
-#pragma unroll
-                    for (int q = 0; q < 6; ++q)
-#pragma unroll
-                        for (uint32_t u = 0; u < U; ++u)
-                            if (s0 + u * L < pl.S) acc[q] += v[q][u];
-                }
-            }
-            for (int q = 0; q < 6; ++q) {
-                for (uint32_t o = 1; o < L; o <<= 1) acc[q] += __shfl_xor(acc[q], o);
-                f[q] = acc[q];
-            }
-        }
-        if (valid && sl == 0) {
+        chunk_sums(s, pl, partial + e.part_pre[c], k, i, valid, f);
+        if (valid && threadIdx.x % L == 0) {
             if (mode == FIN_CORRECT) correct_one(s, i, f, e.cs[c].atime, eta, dt_min, dt_max);
             if (pl.S > 1)
                 for (int q = 0; q < 6; ++q) s.f[q][i] = f[q];
@@ -960,9 +974,58 @@ __global__ __launch_bounds__(BS) void k_blockmin_dirty(Soa s, Ens e) {
     __shared__ double red[BS / 64];
     const uint32_t nd = e.ctl->ndirty;
     for (uint32_t w = blockIdx.x; w < nd; w += gridDim.x) {
-        uint32_t b = e.dirty[w];
+        uint32_t b = e.dirty[3 * w];
         __syncthreads();
         double m = block_tmin(s, b, red);
+        if (threadIdx.x == 0) e.bmin[b] = m;
+    }
+}
+
+// Finish pass of a block step of a single system, one block with active
+// particles per workgroup (the dirty list k_select made: block, first place
+// in the active list, count): the chunk sums and the corrector with
+// NbodyUtils::get_timestep_normal and normalize_dt (Hermite4CPU.cpp:370-411,
+// NbodyUtils.cpp:337-424) of the block's active particles, then the block's
+// min(t + dt), from which k_step_end takes the next block time
+// (Hermite4::next_integration_time, Hermite4.cpp:90-102): one launch for
+// what k_finish and k_blockmin_dirty do in two.  The sums are chunk_sums',
+// so the forces are k_finish's bit for bit.  (An ensemble keeps the two
+// launches: its many blocks with one or two active particles each would
+// put one corrector per workgroup instead of up to BS/L.)
+__global__ __launch_bounds__(BS) void k_finish_blocks(Soa s, Ens e, const uint32_t *move,
+                                                     const double *partial, float eta,
+                                                     double dt_min, double dt_max) {
+    __shared__ double red[BS / 64];
+    __shared__ double tnew[JT];  // t + dt of the block's corrected particles (NaN: not active)
+    const uint32_t nd = e.ctl->ndirty;
+    const ClPlan pl = e.plan[0];
+    const uint32_t L = pl.lanes, per = BS / L;
+    const double T = e.cs[0].atime;
+    for (uint32_t w = blockIdx.x; w < nd; w += gridDim.x) {
+        const uint32_t b = e.dirty[3 * w], k0 = e.dirty[3 * w + 1], cnt = e.dirty[3 * w + 2];
+        // t + dt of the block as it stands, loaded ahead of the corrector
+        const uint32_t j = b * JT + threadIdx.x;
+        const double tdt = s.t[j] + s.dt[j];
+        __syncthreads();  // red and tnew of the previous block have been read
+        tnew[threadIdx.x] = NAN;
+        __syncthreads();
+        for (uint32_t base = 0; base < cnt; base += per) {
+            const uint32_t kk = base + threadIdx.x / L;
+            const bool valid = kk < cnt;
+            const uint32_t k = k0 + (valid ? kk : 0u);
+            const uint32_t i = move[k];
+            double f[6];
+            chunk_sums(s, pl, partial, k, i, valid, f);
+            if (valid && threadIdx.x % L == 0) {
+                const double dtn = correct_one(s, i, f, T, eta, dt_min, dt_max);
+                if (pl.S > 1)
+                    for (int q = 0; q < 6; ++q) s.f[q][i] = f[q];
+                tnew[i - b * JT] = T + dtn;
+            }
+        }
+        __syncthreads();
+        const double tn = tnew[threadIdx.x];
+        const double m = block_min(isnan(tn) ? tdt : tn, red);
         if (threadIdx.x == 0) e.bmin[b] = m;
     }
 }
@@ -1074,7 +1137,7 @@ __global__ __launch_bounds__(BS) void k_step_end(Ens e) {
     __shared__ double red[BS / 64];
     const uint32_t c = blockIdx.x;
     ClState *cs = &e.cs[c];
-    // The dirty list of this step has been consumed by k_blockmin_dirty.
+    // The dirty list of this step has been consumed by the finish pass.
     if (c == 0 && threadIdx.x == 0) e.ctl->ndirty = 0;
     if (cs->done) return;
     const uint32_t b0 = e.off[c] / JT;
@@ -1818,7 +1881,8 @@ int ensure_blockmin(h4h_ctx *ctx) {
 
 // One block step of every live cluster, device-driven: selection with the
 // fused predictor and save-old, work plan, update, finish (chunk sums +
-// corrector), per-block minima of the touched blocks, loop tail.  Optional HIP events bracket the update kernel.
+// corrector), per-block minima of the touched blocks (for one system in the
+// same launch as the finish), loop tail.  Optional HIP events bracket the update kernel.
 int enqueue_step(h4h_ctx *ctx, bool events) {
     int rc;
     if ((rc = launch_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD | PS_DEVTIME | PS_DIRTY,
@@ -1829,10 +1893,17 @@ int enqueue_step(h4h_ctx *ctx, bool events) {
     if (events) H4H_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     if ((rc = launch_update(ctx))) return rc;
     if (events) H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-    if ((rc = launch_finish(ctx, FIN_CORRECT))) return rc;
-    hipLaunchKernelGGL(k_blockmin_dirty, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
-                       ctx->stream, ctx->s, ctx->e);
-    H4H_LAUNCHED();
+    if (ctx->K == 1) {
+        hipLaunchKernelGGL(k_finish_blocks, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
+                           ctx->stream, ctx->s, ctx->e, ctx->move, ctx->partial, ctx->eta,
+                           ctx->dt_min, ctx->dt_max);
+        H4H_LAUNCHED();
+    } else {
+        if ((rc = launch_finish(ctx, FIN_CORRECT))) return rc;
+        hipLaunchKernelGGL(k_blockmin_dirty, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
+                           ctx->stream, ctx->s, ctx->e);
+        H4H_LAUNCHED();
+    }
     hipLaunchKernelGGL(k_step_end, dim3(ctx->K), dim3(BS), 0, ctx->stream, ctx->e);
     H4H_LAUNCHED();
     return H4H_OK;
@@ -2062,7 +2133,7 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
               hipMalloc(&ctx->d_plan, K * sizeof(ClPlan)) == hipSuccess &&
               hipMalloc(&ctx->d_pre, 3 * (size_t)(K + 1) * sizeof(unsigned long long)) == hipSuccess &&
               hipMalloc(&ctx->d_ctl, sizeof(RunCtl)) == hipSuccess &&
-              hipMalloc(&ctx->d_dirty, ctx->nblk * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&ctx->d_dirty, 3 * (size_t)ctx->nblk * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&ctx->d_bmin, ctx->nblk * sizeof(double)) == hipSuccess &&
               hipMalloc(&ctx->d_tune, 64 * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&ctx->hist_t, HIST_CAP * sizeof(double)) == hipSuccess &&
