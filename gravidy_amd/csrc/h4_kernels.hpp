@@ -84,15 +84,29 @@ struct RunCtl {
     double t_end;                  // steps run while itime < t_end
     double t_stop;                 // stop after a step with itime >= t_stop
     unsigned long long max_steps;  // stop after this many steps
-    uint32_t ndirty;               // blocks with active particles this step
+    uint32_t ndirty[2];            // blocks with active particles, by step parity
     uint32_t ndone;                // clusters that have stopped
     uint32_t qdone;                // update workgroups done with the work queue
-    uint32_t pad;
     unsigned long long items;      // update work items this step
     unsigned long long fin_units;  // finish workgroup units this step
     unsigned long long qnext;      // update work queue: items claimed beyond the
                                    // first gridDim.x (zero between launches)
+    // Loop tail of a single system's last block step, left for the next
+    // step's selection to make (the finish pass records the step here):
+    double tail_t;                 // the step's block time
+    unsigned long long tail_steps; // steps done before it
+    uint32_t tail_nact;            // its active count
+    uint32_t tail_pend;            // 1: that tail is still to be made
 };
+
+// The stop test of the reference's loop after a block step made at time T,
+// the next block time being mn (Hermite4CPU_integration.cpp:81,102): an
+// output time reached (t_stop), the step budget spent, ITIME >= t_end, or
+// no particle left ahead of T.
+__host__ __device__ inline bool loop_stops(const RunCtl &ctl, double T, unsigned long long steps,
+                                           double mn) {
+    return T >= ctl.t_stop || steps >= ctl.max_steps || !(T < ctl.t_end) || !(mn > T);
+}
 
 // Cluster tables and per-step scratch, by pointer (device memory).
 struct Ens {
@@ -108,7 +122,8 @@ struct Ens {
     unsigned long long *fin_pre;    // [K+1] exclusive prefix of finish units
     unsigned long long *part_pre;   // [K+1] exclusive prefix of partial doubles
     RunCtl *ctl;
-    uint32_t *dirty;         // [nblk] dirty block list
+    uint32_t *dirty;         // [2][nblk][3] blocks with active particles (block, first
+                             // place in the active list, count), by step parity
     double *bmin;            // [nblk] min(t + dt) per block
     double *hist_t;          // per-step history of cluster 0 (ring)
     uint32_t *hist_n;

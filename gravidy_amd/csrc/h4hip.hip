@@ -316,7 +316,31 @@ enum : uint32_t {
     PS_SAVE_OLD = 4,  // Hermite4::save_old_acc_jrk (Hermite4.cpp:145-153)
     PS_DEVTIME = 8,   // block time = the cluster's ATIME (device step loop)
     PS_DIRTY = 16,    // record blocks holding active particles
+    PS_TAIL = 32,     // single system: make the loop tail the previous step left
 };
+
+// The loop tail of cluster c's block step number `steps` (1-based in this
+// loop) made at block time T with nact active particles, the next block
+// time being mn (Hermite4CPU_integration.cpp:81-124): ITIME and ATIME, the
+// step and interaction counters, the step history of cluster 0, and the
+// stop.  One thread.
+__device__ __forceinline__ void record_tail(const Ens &e, uint32_t c, double T, uint32_t nact,
+                                            unsigned long long steps, double mn, bool stop) {
+    ClState *cs = &e.cs[c];
+    if (c == 0) {
+        e.hist_t[(steps - 1) % e.hist_cap] = T;
+        e.hist_n[(steps - 1) % e.hist_cap] = nact;
+    }
+    cs->itime = T;
+    cs->atime = mn;
+    cs->steps = steps;
+    cs->interactions += (unsigned long long)nact * e.cnt[c];
+    cs->nsteps += nact;
+    if (stop) {
+        cs->done = 1;
+        atomicAdd(&e.ctl->ndone, 1u);
+    }
+}
 
 // The active set of a cluster is appended to its segment of move[] with one
 // atomic per workgroup that has active particles (ballot per wave, prefix
@@ -363,15 +387,66 @@ __device__ __forceinline__ void predict_one(const Soa &s, uint32_t i, double ti,
     if (pv) *pv = b;
 }
 
+// PS_TAIL (one system, device loop): the previous block step's loop tail
+// was left by its finish pass (in ctl->tail_*); every workgroup takes the
+// next block time as the minimum of the block minima and makes the stop
+// decision itself, workgroup 0 records the tail, and the step proceeds at
+// that time unless the loop stops.  This launch reads the tail and the block
+// minima and writes only fields no workgroup of it reads (the stop flag
+// aside: a workgroup that sees it set would have stopped anyway).  Blocks
+// with active particles go to the dirty list of the step's parity (the
+// number of steps before it, mod 2), and workgroup 0 clears the other
+// parity's count for the next step.  Without PS_TAIL the list is parity 0.
 __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, double itime_arg,
                                               uint32_t *move) {
 #pragma clang fp contract(off)
     __shared__ uint32_t wave_off[BS / 64 + 1];
+    __shared__ double wmin[BS / 64];
+    __shared__ double s_time;
+    __shared__ uint32_t s_go, s_par;
     const uint32_t b = blockIdx.x;
     const uint32_t c = cluster_of_block(e, b);
     ClState *cs = &e.cs[c];
-    const bool live = (flags & PS_DEVTIME) ? cs->done == 0 : true;
-    const double itime = (flags & PS_DEVTIME) ? cs->atime : itime_arg;
+    bool live;
+    double itime;
+    uint32_t par = 0;
+    if (flags & PS_TAIL) {
+        RunCtl *ctl = e.ctl;
+        double mn = INFINITY;
+        for (uint32_t k = threadIdx.x; k < e.nblk; k += BS) mn = fmin(mn, e.bmin[k]);
+        for (int o = 32; o > 0; o >>= 1) mn = fmin(mn, __shfl_xor(mn, o));
+        if (__lane_id() == 0) wmin[threadIdx.x >> 6] = mn;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (uint32_t k = 1; k < BS / 64; ++k) mn = fmin(mn, wmin[k]);
+            uint32_t go = 0, p = 0;
+            double T = 0.0;
+            if (cs->done) {
+            } else if (ctl->tail_pend) {
+                const unsigned long long steps = ctl->tail_steps + 1;
+                const bool stop = loop_stops(*ctl, ctl->tail_t, steps, mn);
+                if (b == 0) record_tail(e, 0, ctl->tail_t, ctl->tail_nact, steps, mn, stop);
+                go = !stop;
+                T = mn;
+                p = (uint32_t)(steps & 1);
+            } else {
+                go = 1;
+                T = cs->atime;
+                p = (uint32_t)(cs->steps & 1);
+            }
+            if (go && b == 0) ctl->ndirty[p ^ 1] = 0;
+            s_go = go;
+            s_time = T;
+            s_par = p;
+        }
+        __syncthreads();
+        live = s_go != 0;
+        itime = s_time;
+        par = s_par;
+    } else {
+        live = (flags & PS_DEVTIME) ? cs->done == 0 : true;
+        itime = (flags & PS_DEVTIME) ? cs->atime : itime_arg;
+    }
     const uint32_t i = b * JT + threadIdx.x;
     const bool in = live && (i - e.off[c]) < e.cnt[c];
     bool act = false;
@@ -398,10 +473,11 @@ __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, dou
             }
             wave_off[BS / 64] = total ? atomicAdd(&cs->nact, total) : 0;
             if (total && (flags & PS_DIRTY)) {
-                const uint32_t d = atomicAdd(&e.ctl->ndirty, 1u);
-                e.dirty[3 * d] = b;
-                e.dirty[3 * d + 1] = wave_off[BS / 64];
-                e.dirty[3 * d + 2] = total;
+                const uint32_t d = atomicAdd(&e.ctl->ndirty[par], 1u);
+                uint32_t *dl = e.dirty + (size_t)par * 3 * e.nblk;
+                dl[3 * d] = b;
+                dl[3 * d + 1] = wave_off[BS / 64];
+                dl[3 * d + 2] = total;
             }
         }
         __syncthreads();
@@ -972,7 +1048,7 @@ __device__ __forceinline__ double block_tmin(const Soa &s, uint32_t b, double *r
 // this step (the others did not change).
 __global__ __launch_bounds__(BS) void k_blockmin_dirty(Soa s, Ens e) {
     __shared__ double red[BS / 64];
-    const uint32_t nd = e.ctl->ndirty;
+    const uint32_t nd = e.ctl->ndirty[0];
     for (uint32_t w = blockIdx.x; w < nd; w += gridDim.x) {
         uint32_t b = e.dirty[3 * w];
         __syncthreads();
@@ -982,14 +1058,17 @@ __global__ __launch_bounds__(BS) void k_blockmin_dirty(Soa s, Ens e) {
 }
 
 // Finish pass of a block step of a single system, one block with active
-// particles per workgroup (the dirty list k_select made: block, first place
-// in the active list, count): the chunk sums and the corrector with
+// particles per workgroup (the dirty list k_select made for the step's
+// parity: block, first place in the active list, count): the chunk sums
+// and the corrector with
 // NbodyUtils::get_timestep_normal and normalize_dt (Hermite4CPU.cpp:370-411,
 // NbodyUtils.cpp:337-424) of the block's active particles, then the block's
 // min(t + dt), from which k_step_end takes the next block time
 // (Hermite4::next_integration_time, Hermite4.cpp:90-102): one launch for
 // what k_finish and k_blockmin_dirty do in two.  The sums are chunk_sums',
-// so the forces are k_finish's bit for bit.  (An ensemble keeps the two
+// so the forces are k_finish's bit for bit.  Workgroup 0 records the step
+// for its loop tail (ctl->tail_*), which the next launch that needs the
+// next block time makes.  (An ensemble keeps the two
 // launches: its many blocks with one or two active particles each would
 // put one corrector per workgroup instead of up to BS/L.)
 __global__ __launch_bounds__(BS) void k_finish_blocks(Soa s, Ens e, const uint32_t *move,
@@ -997,12 +1076,27 @@ __global__ __launch_bounds__(BS) void k_finish_blocks(Soa s, Ens e, const uint32
                                                      double dt_min, double dt_max) {
     __shared__ double red[BS / 64];
     __shared__ double tnew[JT];  // t + dt of the block's corrected particles (NaN: not active)
-    const uint32_t nd = e.ctl->ndirty;
+    ClState *cs = &e.cs[0];
+    if (cs->done) return;  // the loop stopped at this step's selection
+    const uint32_t par = (uint32_t)(cs->steps & 1);
+    const uint32_t nd = e.ctl->ndirty[par];
+    const uint32_t *dl = e.dirty + (size_t)par * 3 * e.nblk;
     const ClPlan pl = e.plan[0];
     const uint32_t L = pl.lanes, per = BS / L;
-    const double T = e.cs[0].atime;
+    const double T = cs->atime;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // this step's loop tail, for the next step's selection (k_select
+        // PS_TAIL), k_small_run or k_tail to make; the active count of the
+        // next step starts from zero
+        RunCtl *ctl = e.ctl;
+        ctl->tail_t = T;
+        ctl->tail_nact = cs->nact;
+        ctl->tail_steps = cs->steps;
+        ctl->tail_pend = 1;
+        cs->nact = 0;
+    }
     for (uint32_t w = blockIdx.x; w < nd; w += gridDim.x) {
-        const uint32_t b = e.dirty[3 * w], k0 = e.dirty[3 * w + 1], cnt = e.dirty[3 * w + 2];
+        const uint32_t b = dl[3 * w], k0 = dl[3 * w + 1], cnt = dl[3 * w + 2];
         // t + dt of the block as it stands, loaded ahead of the corrector
         const uint32_t j = b * JT + threadIdx.x;
         const double tdt = s.t[j] + s.dt[j];
@@ -1122,23 +1216,24 @@ __global__ __launch_bounds__(BS) void k_run_setup(Ens e, double t_end, double t_
         ctl.t_end = t_end;
         ctl.t_stop = t_stop;
         ctl.max_steps = max_steps;
-        ctl.ndirty = 0;
+        ctl.ndirty[0] = ctl.ndirty[1] = 0;
+        ctl.tail_pend = 0;
         ctl.ndone = t;
         ctl.items = 0;
         ctl.fin_units = 0;
     }
 }
 
-// Loop tail of each live cluster (one workgroup per cluster): ATIME =
-// min over its blocks, counters, history of cluster 0, and the stop tests:
-// output threshold (Hermite4CPU_integration.cpp:102), step budget, and the
-// loop test ITIME < t_end.
+// Loop tail of each live cluster of an ensemble (one workgroup per
+// cluster): ATIME = min over its blocks, counters, history of cluster 0,
+// and the stop tests: output threshold (Hermite4CPU_integration.cpp:102),
+// step budget, and the loop test ITIME < t_end.
 __global__ __launch_bounds__(BS) void k_step_end(Ens e) {
     __shared__ double red[BS / 64];
     const uint32_t c = blockIdx.x;
     ClState *cs = &e.cs[c];
     // The dirty list of this step has been consumed by the finish pass.
-    if (c == 0 && threadIdx.x == 0) e.ctl->ndirty = 0;
+    if (c == 0 && threadIdx.x == 0) e.ctl->ndirty[0] = 0;
     if (cs->done) return;
     const uint32_t b0 = e.off[c] / JT;
     const uint32_t nb = ntiles(e, c);
@@ -1146,25 +1241,28 @@ __global__ __launch_bounds__(BS) void k_step_end(Ens e) {
     for (uint32_t b = threadIdx.x; b < nb; b += BS) mn = fmin(mn, e.bmin[b0 + b]);
     mn = block_min(mn, red);
     if (threadIdx.x == 0) {
-        const RunCtl &ctl = *e.ctl;
-        double T = cs->atime;
-        uint32_t nact = cs->nact;
-        unsigned long long k = cs->steps;
-        if (c == 0) {
-            e.hist_t[k % e.hist_cap] = T;
-            e.hist_n[k % e.hist_cap] = nact;
-        }
-        cs->itime = T;
-        cs->atime = mn;
-        cs->steps = k + 1;
-        cs->interactions += (unsigned long long)nact * e.cnt[c];
-        cs->nsteps += nact;
+        const double T = cs->atime;
+        const uint32_t nact = cs->nact;
+        const unsigned long long steps = cs->steps + 1;
         cs->nact = 0;
-        // (!(mn > T): no particle left ahead of T, the loop cannot advance.)
-        if (T >= ctl.t_stop || cs->steps >= ctl.max_steps || !(T < ctl.t_end) || !(mn > T)) {
-            cs->done = 1;
-            atomicAdd(&e.ctl->ndone, 1u);
-        }
+        record_tail(e, c, T, nact, steps, mn, loop_stops(*e.ctl, T, steps, mn));
+    }
+}
+
+// The loop tail a single system's last block step left (k_finish_blocks),
+// when no further step of the loop makes it: after one h4h_step.
+__global__ __launch_bounds__(BS) void k_tail(Ens e) {
+    __shared__ double red[BS / 64];
+    RunCtl *ctl = e.ctl;
+    if (e.cs[0].done || !ctl->tail_pend) return;
+    double mn = INFINITY;
+    for (uint32_t b = threadIdx.x; b < e.nblk; b += BS) mn = fmin(mn, e.bmin[b]);
+    mn = block_min(mn, red);
+    if (threadIdx.x == 0) {
+        const unsigned long long steps = ctl->tail_steps + 1;
+        record_tail(e, 0, ctl->tail_t, ctl->tail_nact, steps, mn,
+                    loop_stops(*ctl, ctl->tail_t, steps, mn));
+        ctl->tail_pend = 0;
     }
 }
 
@@ -1285,6 +1383,24 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
         ti = s.t[gi];
         dti = s.dt[gi];
         mf = (float)s.m[gi];
+    }
+    // One system: the loop tail the last multi-workgroup step left
+    // (k_finish_blocks) comes first; its next block time is the minimum of
+    // t + dt, which these threads hold.
+    if (e.K == 1 && e.ctl->tail_pend) {
+        const double Tp = e.ctl->tail_t;
+        const uint32_t np = e.ctl->tail_nact;
+        steps = e.ctl->tail_steps + 1;
+        const double mn = small_min(in ? ti + dti : (double)INFINITY, wmin);
+        const bool stop = loop_stops(*e.ctl, Tp, steps, mn);
+        if (tid == 0) {
+            record_tail(e, 0, Tp, np, steps, mn, stop);
+            e.ctl->tail_pend = 0;
+            e.ctl->ndirty[0] = e.ctl->ndirty[1] = 0;
+        }
+        if (stop) return;  // (the block minima are the finish pass's)
+        hk = (uint32_t)(steps % e.hist_cap);
+        T = mn;
     }
     bool act = in && fabs(T - (ti + dti)) < 2 * DBL_EPSILON;
     uint32_t k_act;  // this particle's place in the active list
@@ -1881,11 +1997,15 @@ int ensure_blockmin(h4h_ctx *ctx) {
 
 // One block step of every live cluster, device-driven: selection with the
 // fused predictor and save-old, work plan, update, finish (chunk sums +
-// corrector), per-block minima of the touched blocks (for one system in the
-// same launch as the finish), loop tail.  Optional HIP events bracket the update kernel.
+// corrector), per-block minima of the touched blocks, loop tail.  One
+// system: three launches, the minima in the finish pass and the loop tail
+// made by the next step's selection (or k_small_run, or k_tail after
+// h4h_step); in a device loop that ends, the step after the last one finds
+// the loop stopped.  Optional HIP events bracket the update kernel.
 int enqueue_step(h4h_ctx *ctx, bool events) {
     int rc;
-    if ((rc = launch_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD | PS_DEVTIME | PS_DIRTY,
+    const uint32_t tail = ctx->K == 1 ? PS_TAIL : 0u;
+    if ((rc = launch_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD | PS_DEVTIME | PS_DIRTY | tail,
                             0.0)))
         return rc;
     // (a single-system context's update kernel makes its own plan)
@@ -1894,6 +2014,7 @@ int enqueue_step(h4h_ctx *ctx, bool events) {
     if ((rc = launch_update(ctx))) return rc;
     if (events) H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
     if (ctx->K == 1) {
+        // (the step's loop tail is made by the next launch that needs it)
         hipLaunchKernelGGL(k_finish_blocks, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
                            ctx->stream, ctx->s, ctx->e, ctx->move, ctx->partial, ctx->eta,
                            ctx->dt_min, ctx->dt_max);
@@ -1903,9 +2024,9 @@ int enqueue_step(h4h_ctx *ctx, bool events) {
         hipLaunchKernelGGL(k_blockmin_dirty, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
                            ctx->stream, ctx->s, ctx->e);
         H4H_LAUNCHED();
+        hipLaunchKernelGGL(k_step_end, dim3(ctx->K), dim3(BS), 0, ctx->stream, ctx->e);
+        H4H_LAUNCHED();
     }
-    hipLaunchKernelGGL(k_step_end, dim3(ctx->K), dim3(BS), 0, ctx->stream, ctx->e);
-    H4H_LAUNCHED();
     return H4H_OK;
 }
 
@@ -2133,7 +2254,7 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
               hipMalloc(&ctx->d_plan, K * sizeof(ClPlan)) == hipSuccess &&
               hipMalloc(&ctx->d_pre, 3 * (size_t)(K + 1) * sizeof(unsigned long long)) == hipSuccess &&
               hipMalloc(&ctx->d_ctl, sizeof(RunCtl)) == hipSuccess &&
-              hipMalloc(&ctx->d_dirty, 3 * (size_t)ctx->nblk * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&ctx->d_dirty, 6 * (size_t)ctx->nblk * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&ctx->d_bmin, ctx->nblk * sizeof(double)) == hipSuccess &&
               hipMalloc(&ctx->d_tune, 64 * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&ctx->hist_t, HIST_CAP * sizeof(double)) == hipSuccess &&
@@ -2688,6 +2809,8 @@ int h4h_step(h4h_ctx *ctx, double itime, uint32_t *nact, double *atime) {
     // The step the device loop would make: k_small_run when it takes it.
     if (small_path(ctx) && (rc = launch_small(ctx, 1))) return rc;
     if ((rc = enqueue_step(ctx, ctx->timing))) return rc;
+    hipLaunchKernelGGL(k_tail, dim3(1), dim3(BS), 0, ctx->stream, ctx->e);
+    H4H_LAUNCHED();
     if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     if ((rc = read_state(ctx))) return rc;
     if (ctx->timing) {
