@@ -699,10 +699,6 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
     const uint32_t js = threadIdx.x / IW;
     uint32_t k = it * IW + il;
     bool act = k < nact;
-    uint32_t i = act ? mv[k] : mv[0];
-    double pi[6];
-    load_predictor(s, i, pi);
-    double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     uint32_t t0 = jt_begin + sc * pl.tpc;
     uint32_t t1 = min(jt_end, t0 + pl.tpc);
     // FP64 wide groups keep their tile as interleaved (position, velocity)
@@ -710,6 +706,22 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
     // instructions' offset field, with no extra address arithmetic (0.5 %
     // on the full step; the mixed kernel runs as fast with two arrays).
     constexpr uint32_t ES = (IW >= 64 && !MIXED) ? 2u : 1u;
+    // The first j tile is requested before the i particles, whose loads wait
+    // for their indices, so the two memory latencies overlap: narrow groups
+    // start its copy into lA (once the previous item is done with lA: tiles
+    // or red), wide groups load it into registers.
+    [[maybe_unused]] double4 ja, jb;
+    if constexpr (IW < 64) {
+        __syncthreads();
+        dma_jtile(s, t0 * JT, lA);
+    } else {
+        ja = s.pp[t0 * JT + threadIdx.x];
+        jb = s.pv[t0 * JT + threadIdx.x];
+    }
+    uint32_t i = act ? mv[k] : mv[0];
+    double pi[6];
+    load_predictor(s, i, pi);
+    double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     auto tile = [&](const double4 *buf, uint32_t tt) {
         const uint32_t j0 = tt * JT;
         const bool self = (i - j0) < (uint32_t)JT;
@@ -727,9 +739,7 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
     // read.  red overlays lA, whose last reader has passed a barrier before
     // reduce_subgroups.
     if constexpr (IW < 64) {
-        __syncthreads();  // the previous item is done with lA (tiles or red)
-        dma_jtile(s, t0 * JT, lA);
-        __syncthreads();
+        __syncthreads();  // the first tile's copy has landed
         for (uint32_t tt = t0; tt < t1; tt += 2) {
             if (tt + 1 < t1) dma_jtile(s, (tt + 1) * JT, lB);
             tile(lA, tt);
@@ -742,7 +752,12 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
     } else {
         // Wide lane groups do 64-256 pairs per lane and tile: the copy is
         // a small share of the time and one buffer keeps registers free.
-        for (uint32_t tt = t0; tt < t1; ++tt) {
+        __syncthreads();  // the previous item is done with lA (tiles or red)
+        lA[ES * threadIdx.x] = ja;
+        lA[ES == 2 ? 2 * threadIdx.x + 1 : JT + threadIdx.x] = jb;
+        __syncthreads();
+        tile(lA, t0);
+        for (uint32_t tt = t0 + 1; tt < t1; ++tt) {
             __syncthreads();
             load_jtile<ES>(s, tt * JT, lA);
             __syncthreads();
