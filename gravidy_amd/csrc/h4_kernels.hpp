@@ -331,23 +331,21 @@ __device__ __forceinline__ void pair_fp64(const double pi[6], double xj,
     fo[5] = fma(mr3, fma(-c, rz, vz), fo[5]);
 }
 
-// 1/sqrt(x) in FP32: v_rsq_f32 estimate and one Newton step
-// y (1 + e/2), e = 1 - x y^2.  (No sqrt(3) scaling here: the FP32 value of
-// sqrt(3) is 1.8e-8 off, a bias every pair term would share.)
-__device__ __forceinline__ float rsqrt_fp32(float x) {
-    float y = __builtin_amdgcn_rsqf(x);
-    float e = fmaf(-x, y * y, 1.0f);
-    return fmaf(y * 0.5f, e, y);
-}
-
 // FP32 pair terms from FP32 differences r = x_j - x_i, v = v_j - v_i:
 // r^2, 1/r, the force and jerk terms, added into the FP32 partial sums
 // t[6].  The caller folds t into FP64 accumulators every few pairs.
+// 1/r is v_rsq_f32 itself (within 1 ulp): a Newton step on it (4 more
+// instructions per pair) changed the N = 262144 Plummer forces' RMS
+// relative error from 2.7e-8 to 1.5e-8 (acceleration) and 2.5e-7 to
+// 2.4e-7 (jerk), both inside the mixed contract, the largest errors not at
+// all (they come from FP32 cancellation in v - c r), and cost 8 % of the
+// kernel's time.  (No sqrt(3) scaling either: the FP32 value of sqrt(3) is
+// 1.8e-8 off, a bias every pair term would share.)
 __device__ __forceinline__ void pair_terms_f32(float rx, float ry, float rz, float vx,
                                                float vy, float vz, float mj, float e2,
                                                float t[6]) {
     float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, e2)));
-    float rinv = rsqrt_fp32(r2);
+    float rinv = __builtin_amdgcn_rsqf(r2);
     float rinv2 = rinv * rinv;
     float mr3 = (mj * rinv) * rinv2;
     float rv = fmaf(rx, vx, fmaf(ry, vy, rz * vz));

@@ -1981,7 +1981,7 @@ bool small_path(const h4h_ctx *ctx) {
 // (r = 0, v = 0) untested: its weight 3 sqrt(3) m / r^3 at r^2 = e2 (FP64;
 // m / r^3 in FP32 for the mixed kernel) must be finite for the largest
 // float mass mmax of the context, and e2 inside the range where the
-// refined hardware rsqrt keeps its accuracy.  Then the pair's terms are
+// hardware rsqrt (refined in FP64) keeps its accuracy.  Then the pair's terms are
 // finite times zero, exact zeros.  mmax is +inf when a mass is not finite.
 bool self_pair_is_zero(int precision, double e2, double mmax) {
     if (precision == H4H_MIXED) {
