@@ -396,6 +396,47 @@ def test_device_loop_matches_host_loop(oracle):
         assert np.array_equal(st, ra["step_time"][5:5 + len(st)])
 
 
+@pytest.mark.parametrize("n,piece", [(4096, 1), (4096, 31), (4096, 33), (1024, 1), (1024, 7),
+                                     (1024, 33)])
+def test_device_loop_in_pieces_is_bitwise(n, piece):
+    """A device loop cut into runs of `piece` block steps by max_steps
+    (runs that end inside or at the end of a captured chunk; N = 1024 with
+    one-workgroup steps in between), with one h4h_step among them, gives
+    the schedule and the state of one uninterrupted run of the same steps,
+    bit for bit: the loop tail of each run's last step is made before the
+    run returns, and the next run starts from it."""
+    r, v = ic.plummer(n, 303)
+    total = 100
+    with Hermite4HIP(r, v, E2, precision="fp64") as h:
+        h.init_acc_jrk()
+        atime = h.init_dt()
+        info, st_ref, sn_ref = h.run(0.0, atime, 1.0, 1.0, total)
+        assert info["steps"] == total
+        ref = h.download()
+    with Hermite4HIP(r, v, E2, precision="fp64") as h:
+        h.init_acc_jrk()
+        atime = h.init_dt()
+        itime, done, st, sn = 0.0, 0, [], []
+        while done < total:
+            if done == 50:
+                nact, nxt = h.step(atime)
+                itime, atime = atime, nxt
+                st.append(itime)
+                sn.append(nact)
+                done += 1
+                continue
+            info, s1, n1 = h.run(itime, atime, 1.0, 1.0, min(piece, total - done))
+            assert info["steps"] == min(piece, total - done) and info["stopped"] == 3
+            itime, atime = info["itime"], info["atime"]
+            st += list(s1)
+            sn += list(n1)
+            done += info["steps"]
+        got = h.download()
+    assert np.array_equal(np.array(st), st_ref) and np.array_equal(np.array(sn), sn_ref)
+    for key in ("r", "v", "f", "t", "dt", "a2", "a3"):
+        assert bits_equal(got[key], ref[key]), key
+
+
 @pytest.mark.parametrize("small_path", [64, 0])
 def test_plummer1024_run_fp64_schedule_and_energy_error_match(oracle, small_path):
     """Fast FP64 kernel over the same run, with the one-workgroup step for
