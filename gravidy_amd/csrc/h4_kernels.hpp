@@ -64,6 +64,12 @@ struct ClState {
     unsigned long long interactions;  // sum of nact * n
     unsigned long long nsteps;        // sum of nact
     unsigned long long small_steps;   // block steps made by the one-workgroup kernel
+    // Loop tail of the cluster's last block step, left by the step's last
+    // launch for the next launch that needs the next block time to make:
+    double tail_t;                    // the step's block time
+    unsigned long long tail_steps;    // steps done before it
+    uint32_t tail_nact;               // its active count
+    uint32_t tail_pend;               // 1: that tail is still to be made
 };
 
 // Work plan of one step for one cluster (computed on the device after the
@@ -91,12 +97,6 @@ struct RunCtl {
     unsigned long long fin_units;  // finish workgroup units this step
     unsigned long long qnext;      // update work queue: items claimed beyond the
                                    // first gridDim.x (zero between launches)
-    // Loop tail of a single system's last block step, left for the next
-    // step's selection to make (the finish pass records the step here):
-    double tail_t;                 // the step's block time
-    unsigned long long tail_steps; // steps done before it
-    uint32_t tail_nact;            // its active count
-    uint32_t tail_pend;            // 1: that tail is still to be made
 };
 
 // The stop test of the reference's loop after a block step made at time T,
@@ -123,7 +123,8 @@ struct Ens {
     unsigned long long *part_pre;   // [K+1] exclusive prefix of partial doubles
     RunCtl *ctl;
     uint32_t *dirty;         // [2][nblk][3] blocks with active particles (block, first
-                             // place in the active list, count), by step parity
+                             // place in the cluster's active list, count), by step
+                             // parity (the step's slot in its captured chunk, mod 2)
     double *bmin;            // [nblk] min(t + dt) per block
     double *hist_t;          // per-step history of cluster 0 (ring)
     uint32_t *hist_n;

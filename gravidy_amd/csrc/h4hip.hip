@@ -316,7 +316,7 @@ enum : uint32_t {
     PS_SAVE_OLD = 4,  // Hermite4::save_old_acc_jrk (Hermite4.cpp:145-153)
     PS_DEVTIME = 8,   // block time = the cluster's ATIME (device step loop)
     PS_DIRTY = 16,    // record blocks holding active particles
-    PS_TAIL = 32,     // single system: make the loop tail the previous step left
+    PS_TAIL = 32,     // make the loop tail the previous step left (device loop)
 };
 
 // The loop tail of cluster c's block step number `steps` (1-based in this
@@ -387,62 +387,57 @@ __device__ __forceinline__ void predict_one(const Soa &s, uint32_t i, double ti,
     if (pv) *pv = b;
 }
 
-// PS_TAIL (one system, device loop): the previous block step's loop tail
-// was left by its finish pass (in ctl->tail_*); every workgroup takes the
-// next block time as the minimum of the block minima and makes the stop
-// decision itself, workgroup 0 records the tail, and the step proceeds at
-// that time unless the loop stops.  This launch reads the tail and the block
-// minima and writes only fields no workgroup of it reads (the stop flag
-// aside: a workgroup that sees it set would have stopped anyway).  Blocks
-// with active particles go to the dirty list of the step's parity (the
-// number of steps before it, mod 2), and workgroup 0 clears the other
-// parity's count for the next step.  Without PS_TAIL the list is parity 0.
+// PS_TAIL (device loop): each cluster's last block step left its loop tail
+// in its ClState (tail_*); every workgroup of the cluster takes the next
+// block time as the minimum of the cluster's block minima and makes the
+// stop decision itself, the cluster's first workgroup records the tail, and
+// the step proceeds at that time unless the cluster's loop stops.  This
+// launch reads the tail records and the block minima and writes only fields
+// no workgroup of it reads (the stop flag aside: a workgroup that sees it
+// set would have stopped anyway).  Blocks with active particles go to the
+// dirty list `par` (the step's slot in its captured chunk, mod 2), and
+// workgroup 0 clears the other list's count for the next step.
 __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, double itime_arg,
-                                              uint32_t *move) {
+                                              uint32_t par, uint32_t *move) {
 #pragma clang fp contract(off)
     __shared__ uint32_t wave_off[BS / 64 + 1];
     __shared__ double wmin[BS / 64];
     __shared__ double s_time;
-    __shared__ uint32_t s_go, s_par;
+    __shared__ uint32_t s_go;
     const uint32_t b = blockIdx.x;
     const uint32_t c = cluster_of_block(e, b);
     ClState *cs = &e.cs[c];
     bool live;
     double itime;
-    uint32_t par = 0;
     if (flags & PS_TAIL) {
-        RunCtl *ctl = e.ctl;
+        const uint32_t b0 = e.off[c] / JT, nb = ntiles(e, c);
         double mn = INFINITY;
-        for (uint32_t k = threadIdx.x; k < e.nblk; k += BS) mn = fmin(mn, e.bmin[k]);
+        for (uint32_t k = threadIdx.x; k < nb; k += BS) mn = fmin(mn, e.bmin[b0 + k]);
         for (int o = 32; o > 0; o >>= 1) mn = fmin(mn, __shfl_xor(mn, o));
         if (__lane_id() == 0) wmin[threadIdx.x >> 6] = mn;
         __syncthreads();
         if (threadIdx.x == 0) {
             for (uint32_t k = 1; k < BS / 64; ++k) mn = fmin(mn, wmin[k]);
-            uint32_t go = 0, p = 0;
+            uint32_t go = 0;
             double T = 0.0;
             if (cs->done) {
-            } else if (ctl->tail_pend) {
-                const unsigned long long steps = ctl->tail_steps + 1;
-                const bool stop = loop_stops(*ctl, ctl->tail_t, steps, mn);
-                if (b == 0) record_tail(e, 0, ctl->tail_t, ctl->tail_nact, steps, mn, stop);
+            } else if (cs->tail_pend) {
+                const unsigned long long steps = cs->tail_steps + 1;
+                const bool stop = loop_stops(*e.ctl, cs->tail_t, steps, mn);
+                if (b == b0) record_tail(e, c, cs->tail_t, cs->tail_nact, steps, mn, stop);
                 go = !stop;
                 T = mn;
-                p = (uint32_t)(steps & 1);
             } else {
                 go = 1;
                 T = cs->atime;
-                p = (uint32_t)(cs->steps & 1);
             }
-            if (go && b == 0) ctl->ndirty[p ^ 1] = 0;
+            if (b == 0) e.ctl->ndirty[par ^ 1] = 0;
             s_go = go;
             s_time = T;
-            s_par = p;
         }
         __syncthreads();
         live = s_go != 0;
         itime = s_time;
-        par = s_par;
     } else {
         live = (flags & PS_DEVTIME) ? cs->done == 0 : true;
         itime = (flags & PS_DEVTIME) ? cs->atime : itime_arg;
@@ -1060,12 +1055,25 @@ __device__ __forceinline__ double block_tmin(const Soa &s, uint32_t b, double *r
 }
 
 // Recomputes the per-block minima of the blocks that had active particles
-// this step (the others did not change).
-__global__ __launch_bounds__(BS) void k_blockmin_dirty(Soa s, Ens e) {
+// this step (dirty list `par`; the others did not change), and records each
+// live cluster's step for its loop tail, which the next launch that needs
+// the cluster's next block time makes (k_select PS_TAIL, k_small_run); the
+// cluster's active count of the next step starts from zero.
+__global__ __launch_bounds__(BS) void k_blockmin_dirty(Soa s, Ens e, uint32_t par) {
     __shared__ double red[BS / 64];
-    const uint32_t nd = e.ctl->ndirty[0];
+    for (uint32_t c = blockIdx.x * BS + threadIdx.x; c < e.K; c += gridDim.x * BS) {
+        ClState *cs = &e.cs[c];
+        if (cs->done) continue;
+        cs->tail_t = cs->atime;
+        cs->tail_nact = cs->nact;
+        cs->tail_steps = cs->steps;
+        cs->tail_pend = 1;
+        cs->nact = 0;
+    }
+    const uint32_t nd = e.ctl->ndirty[par];
+    const uint32_t *dl = e.dirty + (size_t)par * 3 * e.nblk;
     for (uint32_t w = blockIdx.x; w < nd; w += gridDim.x) {
-        uint32_t b = e.dirty[3 * w];
+        uint32_t b = dl[3 * w];
         __syncthreads();
         double m = block_tmin(s, b, red);
         if (threadIdx.x == 0) e.bmin[b] = m;
@@ -1078,22 +1086,21 @@ __global__ __launch_bounds__(BS) void k_blockmin_dirty(Soa s, Ens e) {
 // and the corrector with
 // NbodyUtils::get_timestep_normal and normalize_dt (Hermite4CPU.cpp:370-411,
 // NbodyUtils.cpp:337-424) of the block's active particles, then the block's
-// min(t + dt), from which k_step_end takes the next block time
+// min(t + dt), from which the loop tail takes the next block time
 // (Hermite4::next_integration_time, Hermite4.cpp:90-102): one launch for
 // what k_finish and k_blockmin_dirty do in two.  The sums are chunk_sums',
 // so the forces are k_finish's bit for bit.  Workgroup 0 records the step
-// for its loop tail (ctl->tail_*), which the next launch that needs the
+// for its loop tail (ClState tail_*), which the next launch that needs the
 // next block time makes.  (An ensemble keeps the two
 // launches: its many blocks with one or two active particles each would
 // put one corrector per workgroup instead of up to BS/L.)
 __global__ __launch_bounds__(BS) void k_finish_blocks(Soa s, Ens e, const uint32_t *move,
                                                      const double *partial, float eta,
-                                                     double dt_min, double dt_max) {
+                                                     double dt_min, double dt_max, uint32_t par) {
     __shared__ double red[BS / 64];
     __shared__ double tnew[JT];  // t + dt of the block's corrected particles (NaN: not active)
     ClState *cs = &e.cs[0];
     if (cs->done) return;  // the loop stopped at this step's selection
-    const uint32_t par = (uint32_t)(cs->steps & 1);
     const uint32_t nd = e.ctl->ndirty[par];
     const uint32_t *dl = e.dirty + (size_t)par * 3 * e.nblk;
     const ClPlan pl = e.plan[0];
@@ -1103,11 +1110,10 @@ __global__ __launch_bounds__(BS) void k_finish_blocks(Soa s, Ens e, const uint32
         // this step's loop tail, for the next step's selection (k_select
         // PS_TAIL), k_small_run or k_tail to make; the active count of the
         // next step starts from zero
-        RunCtl *ctl = e.ctl;
-        ctl->tail_t = T;
-        ctl->tail_nact = cs->nact;
-        ctl->tail_steps = cs->steps;
-        ctl->tail_pend = 1;
+        cs->tail_t = T;
+        cs->tail_nact = cs->nact;
+        cs->tail_steps = cs->steps;
+        cs->tail_pend = 1;
         cs->nact = 0;
     }
     for (uint32_t w = blockIdx.x; w < nd; w += gridDim.x) {
@@ -1218,6 +1224,7 @@ __global__ __launch_bounds__(BS) void k_run_setup(Ens e, double t_end, double t_
         cs.nsteps = 0;
         cs.small_steps = 0;
         cs.nact = 0;
+        cs.tail_pend = 0;
         cs.done = (cs.itime < t_end) ? 0u : 1u;
         nd += cs.done;
     }
@@ -1232,35 +1239,9 @@ __global__ __launch_bounds__(BS) void k_run_setup(Ens e, double t_end, double t_
         ctl.t_stop = t_stop;
         ctl.max_steps = max_steps;
         ctl.ndirty[0] = ctl.ndirty[1] = 0;
-        ctl.tail_pend = 0;
         ctl.ndone = t;
         ctl.items = 0;
         ctl.fin_units = 0;
-    }
-}
-
-// Loop tail of each live cluster of an ensemble (one workgroup per
-// cluster): ATIME = min over its blocks, counters, history of cluster 0,
-// and the stop tests: output threshold (Hermite4CPU_integration.cpp:102),
-// step budget, and the loop test ITIME < t_end.
-__global__ __launch_bounds__(BS) void k_step_end(Ens e) {
-    __shared__ double red[BS / 64];
-    const uint32_t c = blockIdx.x;
-    ClState *cs = &e.cs[c];
-    // The dirty list of this step has been consumed by the finish pass.
-    if (c == 0 && threadIdx.x == 0) e.ctl->ndirty[0] = 0;
-    if (cs->done) return;
-    const uint32_t b0 = e.off[c] / JT;
-    const uint32_t nb = ntiles(e, c);
-    double mn = INFINITY;
-    for (uint32_t b = threadIdx.x; b < nb; b += BS) mn = fmin(mn, e.bmin[b0 + b]);
-    mn = block_min(mn, red);
-    if (threadIdx.x == 0) {
-        const double T = cs->atime;
-        const uint32_t nact = cs->nact;
-        const unsigned long long steps = cs->steps + 1;
-        cs->nact = 0;
-        record_tail(e, c, T, nact, steps, mn, loop_stops(*e.ctl, T, steps, mn));
     }
 }
 
@@ -1268,16 +1249,16 @@ __global__ __launch_bounds__(BS) void k_step_end(Ens e) {
 // when no further step of the loop makes it: after one h4h_step.
 __global__ __launch_bounds__(BS) void k_tail(Ens e) {
     __shared__ double red[BS / 64];
-    RunCtl *ctl = e.ctl;
-    if (e.cs[0].done || !ctl->tail_pend) return;
+    ClState *cs = &e.cs[0];
+    if (cs->done || !cs->tail_pend) return;
     double mn = INFINITY;
     for (uint32_t b = threadIdx.x; b < e.nblk; b += BS) mn = fmin(mn, e.bmin[b]);
     mn = block_min(mn, red);
     if (threadIdx.x == 0) {
-        const unsigned long long steps = ctl->tail_steps + 1;
-        record_tail(e, 0, ctl->tail_t, ctl->tail_nact, steps, mn,
-                    loop_stops(*ctl, ctl->tail_t, steps, mn));
-        ctl->tail_pend = 0;
+        const unsigned long long steps = cs->tail_steps + 1;
+        record_tail(e, 0, cs->tail_t, cs->tail_nact, steps, mn,
+                    loop_stops(*e.ctl, cs->tail_t, steps, mn));
+        cs->tail_pend = 0;
     }
 }
 
@@ -1341,7 +1322,7 @@ __device__ __forceinline__ double small_min(double v, double *wmin) {
 // step: predictor of all particles at the block time T, the particles with
 // t + dt = T (Hermite4::find_particles_to_move), their old forces, the pair
 // sums, the corrector and new step, the next block time min(t + dt) and the
-// loop tail of k_step_end.  The active set of the next step is counted
+// loop tail (the record_tail of each step).  The active set of the next step is counted
 // before its predictor; when it is empty or larger than max_act the launch
 // ends and leaves that step to the multi-workgroup kernels.
 //
@@ -1399,18 +1380,19 @@ __global__ __launch_bounds__(SB) void k_small_run(Soa s, Ens e, uint32_t max_run
         dti = s.dt[gi];
         mf = (float)s.m[gi];
     }
-    // One system: the loop tail the last multi-workgroup step left
-    // (k_finish_blocks) comes first; its next block time is the minimum of
-    // t + dt, which these threads hold.
-    if (e.K == 1 && e.ctl->tail_pend) {
-        const double Tp = e.ctl->tail_t;
-        const uint32_t np = e.ctl->tail_nact;
-        steps = e.ctl->tail_steps + 1;
+    // The loop tail the cluster's last multi-workgroup step left comes
+    // first; its next block time is the minimum of t + dt, which these
+    // threads hold.  (The dirty lists are cleared for the steps that follow
+    // this launch, whose parities need not continue the last step's.)
+    if (cs->tail_pend) {
+        const double Tp = cs->tail_t;
+        const uint32_t np = cs->tail_nact;
+        steps = cs->tail_steps + 1;
         const double mn = small_min(in ? ti + dti : (double)INFINITY, wmin);
         const bool stop = loop_stops(*e.ctl, Tp, steps, mn);
         if (tid == 0) {
-            record_tail(e, 0, Tp, np, steps, mn, stop);
-            e.ctl->tail_pend = 0;
+            record_tail(e, c, Tp, np, steps, mn, stop);
+            cs->tail_pend = 0;
             e.ctl->ndirty[0] = e.ctl->ndirty[1] = 0;
         }
         if (stop) return;  // (the block minima are the finish pass's)
@@ -1782,6 +1764,8 @@ namespace {
 constexpr uint32_t RUN_CHUNK = 32;    // block steps per captured graph
 constexpr uint32_t SMALL_RUN = 255;   // k_small_run steps per launch
 constexpr uint32_t SMALL_SLOTS = 8;   // (k_small_run + block step) pairs per captured graph
+static_assert(RUN_CHUNK % 2 == 0 && SMALL_SLOTS % 2 == 0,
+              "a step's dirty list is its slot in the chunk mod 2; chunks follow each other");
 constexpr uint32_t SMALL_ACT = 64;    // default largest active set k_small_run takes (one system)
 constexpr uint32_t HIST_CAP = 2048;   // per-step history ring of cluster 0
 static_assert(HIST_CAP >= RUN_CHUNK && HIST_CAP >= SMALL_SLOTS * (SMALL_RUN + 1),
@@ -1940,9 +1924,9 @@ int launch_finish(h4h_ctx *ctx, uint32_t mode) {
     return H4H_OK;
 }
 
-int launch_select(h4h_ctx *ctx, uint32_t flags, double itime) {
+int launch_select(h4h_ctx *ctx, uint32_t flags, double itime, uint32_t par = 0) {
     hipLaunchKernelGGL(k_select, dim3(ctx->nblk), dim3(BS), 0, ctx->stream, ctx->s, ctx->e,
-                       flags, itime, ctx->move);
+                       flags, itime, par, ctx->move);
     H4H_LAUNCHED();
     return H4H_OK;
 }
@@ -2011,17 +1995,19 @@ int ensure_blockmin(h4h_ctx *ctx) {
 }
 
 // One block step of every live cluster, device-driven: selection with the
-// fused predictor and save-old, work plan, update, finish (chunk sums +
-// corrector), per-block minima of the touched blocks, loop tail.  One
-// system: three launches, the minima in the finish pass and the loop tail
-// made by the next step's selection (or k_small_run, or k_tail after
-// h4h_step); in a device loop that ends, the step after the last one finds
-// the loop stopped.  Optional HIP events bracket the update kernel.
-int enqueue_step(h4h_ctx *ctx, bool events) {
+// fused predictor and save-old (after the loop tail of each cluster's last
+// step), work plan, update, finish (chunk sums + corrector) and the
+// per-block minima of the touched blocks; each cluster's loop tail is made
+// by the next launch that needs its next block time (the next step's
+// selection, k_small_run, or k_tail after h4h_step), so in a device loop
+// that ends, the step after the last one finds the loop stopped.  One
+// system: three launches (the minima in the finish pass); an ensemble: five.
+// par: the step's slot in its captured chunk, mod 2 (its dirty list).
+// Optional HIP events bracket the update kernel.
+int enqueue_step(h4h_ctx *ctx, bool events, uint32_t par) {
     int rc;
-    const uint32_t tail = ctx->K == 1 ? PS_TAIL : 0u;
-    if ((rc = launch_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD | PS_DEVTIME | PS_DIRTY | tail,
-                            0.0)))
+    if ((rc = launch_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD | PS_DEVTIME | PS_DIRTY | PS_TAIL,
+                            0.0, par)))
         return rc;
     // (a single-system context's update kernel makes its own plan)
     if (ctx->K > 1 && (rc = launch_plan(ctx))) return rc;
@@ -2029,17 +2015,14 @@ int enqueue_step(h4h_ctx *ctx, bool events) {
     if ((rc = launch_update(ctx))) return rc;
     if (events) H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
     if (ctx->K == 1) {
-        // (the step's loop tail is made by the next launch that needs it)
         hipLaunchKernelGGL(k_finish_blocks, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
                            ctx->stream, ctx->s, ctx->e, ctx->move, ctx->partial, ctx->eta,
-                           ctx->dt_min, ctx->dt_max);
+                           ctx->dt_min, ctx->dt_max, par);
         H4H_LAUNCHED();
     } else {
         if ((rc = launch_finish(ctx, FIN_CORRECT))) return rc;
         hipLaunchKernelGGL(k_blockmin_dirty, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
-                           ctx->stream, ctx->s, ctx->e);
-        H4H_LAUNCHED();
-        hipLaunchKernelGGL(k_step_end, dim3(ctx->K), dim3(BS), 0, ctx->stream, ctx->e);
+                           ctx->stream, ctx->s, ctx->e, par);
         H4H_LAUNCHED();
     }
     return H4H_OK;
@@ -2053,9 +2036,9 @@ int enqueue_chunk(h4h_ctx *ctx) {
     int rc = H4H_OK;
     if (small_path(ctx)) {
         for (uint32_t k = 0; k < SMALL_SLOTS && rc == H4H_OK; ++k)
-            if ((rc = launch_small(ctx, SMALL_RUN)) == H4H_OK) rc = enqueue_step(ctx, false);
+            if ((rc = launch_small(ctx, SMALL_RUN)) == H4H_OK) rc = enqueue_step(ctx, false, k & 1);
     } else {
-        for (uint32_t k = 0; k < RUN_CHUNK && rc == H4H_OK; ++k) rc = enqueue_step(ctx, false);
+        for (uint32_t k = 0; k < RUN_CHUNK && rc == H4H_OK; ++k) rc = enqueue_step(ctx, false, k & 1);
     }
     return rc;
 }
@@ -2823,7 +2806,7 @@ int h4h_step(h4h_ctx *ctx, double itime, uint32_t *nact, double *atime) {
     H4H_LAUNCHED();
     // The step the device loop would make: k_small_run when it takes it.
     if (small_path(ctx) && (rc = launch_small(ctx, 1))) return rc;
-    if ((rc = enqueue_step(ctx, ctx->timing))) return rc;
+    if ((rc = enqueue_step(ctx, ctx->timing, 0))) return rc;
     hipLaunchKernelGGL(k_tail, dim3(1), dim3(BS), 0, ctx->stream, ctx->e);
     H4H_LAUNCHED();
     if (ctx->timing) H4H_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
