@@ -126,6 +126,10 @@ struct Ens {
                              // place in the cluster's active list, count), by step
                              // parity (the step's slot in its captured chunk, mod 2)
     double *bmin;            // [nblk] min(t + dt) per block
+    double4 *arec;           // [4][nblk * JT] the step's active particles in the order
+                             // of the active list (index off[c] + k): predictor
+                             // (pp, pv), then old forces and step ((a, dt), (jerk, 0));
+                             // written by the selection of a device-loop step
     double *hist_t;          // per-step history of cluster 0 (ring)
     uint32_t *hist_n;
     uint32_t hist_cap;

@@ -317,6 +317,8 @@ enum : uint32_t {
     PS_DEVTIME = 8,   // block time = the cluster's ATIME (device step loop)
     PS_DIRTY = 16,    // record blocks holding active particles
     PS_TAIL = 32,     // make the loop tail the previous step left (device loop)
+    PS_COMPACT = 64,  // the active particles' records in Ens::arec (with
+                      // PS_PREDICT | PS_SELECT | PS_SAVE_OLD)
 };
 
 // The loop tail of cluster c's block step number `steps` (1-based in this
@@ -405,12 +407,46 @@ __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, dou
     __shared__ double s_time;
     __shared__ uint32_t s_go;
     const uint32_t b = blockIdx.x;
-    const uint32_t c = cluster_of_block(e, b);
+    const uint32_t i = b * JT + threadIdx.x;  // inside the padded arrays
+    const bool single = e.K == 1;
+    const uint32_t c = single ? 0u : cluster_of_block(e, b);
     ClState *cs = &e.cs[c];
+    const bool devloop = (flags & (PS_TAIL | PS_DEVTIME)) != 0;
+    // The particle's state does not depend on the block time, so its loads
+    // go out before the loop tail's minimum is known and their latency
+    // overlaps that of the cluster record and the block minima.  A single
+    // system loads it at once (the step slots of a captured chunk that find
+    // the loop stopped load it for nothing); an ensemble, whose stopped
+    // clusters can be most of it, first reads whether the cluster stopped.
+    const bool load = single || !devloop || cs->done == 0;
+    const bool need_f = (flags & (PS_PREDICT | PS_SAVE_OLD)) != 0;
+    double ti = 0.0, dti = 0.0, f[6], r[3], v[3];
+    float mf = 0.f;
+    for (int k = 0; k < 6; ++k) f[k] = 0.0;
+    for (int k = 0; k < 3; ++k) r[k] = v[k] = 0.0;
+    if (load) {
+        if (flags & (PS_PREDICT | PS_SELECT)) ti = s.t[i];
+        if (flags & PS_SELECT) dti = s.dt[i];
+        if (need_f)
+            for (int k = 0; k < 6; ++k) f[k] = s.f[k][i];
+        if (flags & PS_PREDICT) {
+            for (int k = 0; k < 3; ++k) {
+                r[k] = s.r[k][i];
+                v[k] = s.v[k][i];
+            }
+            mf = (float)s.m[i];
+        }
+    }
     bool live;
     double itime;
     if (flags & PS_TAIL) {
-        const uint32_t b0 = e.off[c] / JT, nb = ntiles(e, c);
+        // the cluster record, read ahead of the barrier (this launch writes
+        // none of these fields that another workgroup reads)
+        const uint32_t done = cs->done, pend = cs->tail_pend;
+        const double tail_t = cs->tail_t, atime = cs->atime;
+        const unsigned long long tail_steps = cs->tail_steps;
+        const uint32_t tail_nact = cs->tail_nact;
+        const uint32_t b0 = single ? 0u : e.off[c] / JT, nb = single ? e.nblk : ntiles(e, c);
         double mn = INFINITY;
         for (uint32_t k = threadIdx.x; k < nb; k += BS) mn = fmin(mn, e.bmin[b0 + k]);
         for (int o = 32; o > 0; o >>= 1) mn = fmin(mn, __shfl_xor(mn, o));
@@ -420,16 +456,16 @@ __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, dou
             for (uint32_t k = 1; k < BS / 64; ++k) mn = fmin(mn, wmin[k]);
             uint32_t go = 0;
             double T = 0.0;
-            if (cs->done) {
-            } else if (cs->tail_pend) {
-                const unsigned long long steps = cs->tail_steps + 1;
-                const bool stop = loop_stops(*e.ctl, cs->tail_t, steps, mn);
-                if (b == b0) record_tail(e, c, cs->tail_t, cs->tail_nact, steps, mn, stop);
+            if (done) {
+            } else if (pend) {
+                const unsigned long long steps = tail_steps + 1;
+                const bool stop = loop_stops(*e.ctl, tail_t, steps, mn);
+                if (b == b0) record_tail(e, c, tail_t, tail_nact, steps, mn, stop);
                 go = !stop;
                 T = mn;
             } else {
                 go = 1;
-                T = cs->atime;
+                T = atime;
             }
             if (b == 0) e.ctl->ndirty[par ^ 1] = 0;
             s_go = go;
@@ -442,14 +478,17 @@ __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, dou
         live = (flags & PS_DEVTIME) ? cs->done == 0 : true;
         itime = (flags & PS_DEVTIME) ? cs->atime : itime_arg;
     }
-    const uint32_t i = b * JT + threadIdx.x;
     const bool in = live && (i - e.off[c]) < e.cnt[c];
     bool act = false;
+    double4 pa = {}, pb = {};
     if (in) {
-        double ti = s.t[i];
-        if (flags & PS_PREDICT) predict_one(s, i, ti, itime);
+        if (flags & PS_PREDICT) {
+            predict_vals(r, v, f, itime - ti, mf, pa, pb);
+            s.pp[i] = pa;
+            s.pv[i] = pb;
+        }
         if (flags & PS_SELECT) {
-            double tmp_time = ti + s.dt[i];
+            double tmp_time = ti + dti;
             act = fabs(itime - tmp_time) < 2 * DBL_EPSILON;
         }
     }
@@ -478,9 +517,18 @@ __global__ __launch_bounds__(BS) void k_select(Soa s, Ens e, uint32_t flags, dou
         __syncthreads();
         if (act) {
             unsigned long long below = mask & ((1ull << lane) - 1ull);
-            move[e.off[c] + wave_off[BS / 64] + wave_off[w] + (uint32_t)__popcll(below)] = i;
+            const uint32_t slot =
+                e.off[c] + wave_off[BS / 64] + wave_off[w] + (uint32_t)__popcll(below);
+            move[slot] = i;
             if (flags & PS_SAVE_OLD)
-                for (int k = 0; k < 6; ++k) s.old[k][i] = s.f[k][i];
+                for (int k = 0; k < 6; ++k) s.old[k][i] = f[k];
+            if (flags & PS_COMPACT) {
+                const size_t np = (size_t)e.nblk * JT;
+                e.arec[slot] = pa;
+                e.arec[np + slot] = pb;
+                e.arec[2 * np + slot] = make_double4(f[0], f[1], f[2], dti);
+                e.arec[3 * np + slot] = make_double4(f[3], f[4], f[5], 0.0);
+            }
         }
     }
 }
@@ -688,8 +736,9 @@ __global__ __launch_bounds__(BS) void k_update_exact(Soa s, Ens e, const uint32_
 template <int IW, bool MIXED>
 __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint32_t jt_begin,
                                             uint32_t jt_end, uint32_t it, uint32_t sc,
-                                            const uint32_t *mv, uint32_t nact, double *partial,
-                                            double e2, double4 *lA, double4 *lB, double *red) {
+                                            const uint32_t *mv, const double4 *cr, size_t np,
+                                            uint32_t nact, double *partial, double e2,
+                                            double4 *lA, double4 *lB, double *red) {
     const uint32_t il = threadIdx.x % IW;
     const uint32_t js = threadIdx.x / IW;
     uint32_t k = it * IW + il;
@@ -713,9 +762,23 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
         ja = s.pp[t0 * JT + threadIdx.x];
         jb = s.pv[t0 * JT + threadIdx.x];
     }
+    // The i particles' predictors: from the selection's records in
+    // active-list order when it wrote them (cr; then their loads wait for
+    // no index), else through the active list.
     uint32_t i = act ? mv[k] : mv[0];
     double pi[6];
-    load_predictor(s, i, pi);
+    if (cr) {
+        const uint32_t kk = act ? k : 0u;
+        const double4 a = cr[kk], b = cr[np + kk];
+        pi[0] = a.x;
+        pi[1] = a.y;
+        pi[2] = a.z;
+        pi[3] = b.x;
+        pi[4] = b.y;
+        pi[5] = b.z;
+    } else {
+        load_predictor(s, i, pi);
+    }
     double fo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     auto tile = [&](const double4 *buf, uint32_t tt) {
         const uint32_t j0 = tt * JT;
@@ -776,7 +839,7 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
 // so one launch serves every active-set size.
 template <bool MIXED>
 __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const uint32_t *move,
-                                             double *partial, double e2) {
+                                             double *partial, double e2, uint32_t compact) {
     __shared__ double4 lA[2 * JT];
     __shared__ double4 lB[2 * JT];
     static_assert(BS * 6 * sizeof(double) <= sizeof(lA), "red overlays lA");
@@ -803,19 +866,25 @@ __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const u
         const uint32_t jt_begin = e.off[c] / JT;
         const uint32_t jt_end = jt_begin + ntiles(e, c);
         const uint32_t *mv = move + e.off[c];
+        const double4 *cr = compact ? e.arec + e.off[c] : nullptr;
+        const size_t np = (size_t)e.nblk * JT;
         double *part = partial + (single ? 0ull : e.part_pre[c]);
         switch (pl.iw) {
             case 4:
-                update_item<4, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, lA, lB, red);
+                update_item<4, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, cr, np, nact, part, e2,
+                                      lA, lB, red);
                 break;
             case 16:
-                update_item<16, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, lA, lB, red);
+                update_item<16, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, cr, np, nact, part, e2,
+                                       lA, lB, red);
                 break;
             case 64:
-                update_item<64, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, lA, lB, red);
+                update_item<64, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, cr, np, nact, part, e2,
+                                       lA, lB, red);
                 break;
             default:
-                update_item<256, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, nact, part, e2, lA, lB, red);
+                update_item<256, MIXED>(s, pl, jt_begin, jt_end, it, sc, mv, cr, np, nact, part, e2,
+                                        lA, lB, red);
                 break;
         }
         if (total <= gridDim.x) break;
@@ -837,14 +906,18 @@ __device__ __forceinline__ void update_items(const Soa &s, const Ens &e, const u
 // per SIMD); each alone needs <= 126, so the FP64 kernel is held to 128
 // (4 waves per SIMD; a few per-item values spill to scratch, none inside
 // the tile or pair loops).  The mixed kernel fits in 120 by itself.
+// compact: the step's selection wrote the active particles' records
+// (k_select PS_COMPACT).
 __global__ __launch_bounds__(BS, 4) void k_update_fp64(Soa s, Ens e, const uint32_t *move,
-                                                      double *partial, double e2) {
-    update_items<false>(s, e, move, partial, e2);
+                                                      double *partial, double e2,
+                                                      uint32_t compact) {
+    update_items<false>(s, e, move, partial, e2, compact);
 }
 
 __global__ __launch_bounds__(BS) void k_update_mixed(Soa s, Ens e, const uint32_t *move,
-                                                    double *partial, double e2) {
-    update_items<true>(s, e, move, partial, e2);
+                                                    double *partial, double e2,
+                                                    uint32_t compact) {
+    update_items<true>(s, e, move, partial, e2, compact);
 }
 
 // ---------------------------------------------------------------------------
@@ -925,14 +998,13 @@ __device__ __forceinline__ double correct_vals(const double o[6], const double f
 // The corrector of particle i at block time itime, given its new forces
 // f[6] (old forces in s.old, predictor in s.pp / s.pv), on the state in
 // HBM.  Returns the new dt.
-__device__ __forceinline__ double correct_one(const Soa &s, uint32_t i, const double f[6],
-                                              double itime, float eta, double dt_min,
-                                              double dt_max) {
-    double o[6], pr[6], rn[3], vn[3], a2[3], a3[3];
-    for (int c = 0; c < 6; ++c) o[c] = s.old[c][i];
-    load_predictor(s, i, pr);
-    const double dt_new = correct_vals(o, f, pr, s.dt[i], itime, eta, dt_min, dt_max, rn, vn,
-                                       a2, a3);
+__device__ __forceinline__ double correct_store(const Soa &s, uint32_t i, const double o[6],
+                                                const double pr[6], double dt1, const double f[6],
+                                                double itime, float eta, double dt_min,
+                                                double dt_max) {
+    double rn[3], vn[3], a2[3], a3[3];
+    const double dt_new =
+        correct_vals(o, f, pr, dt1, itime, eta, dt_min, dt_max, rn, vn, a2, a3);
     for (int c = 0; c < 3; ++c) {
         s.r[c][i] = rn[c];
         s.v[c][i] = vn[c];
@@ -942,6 +1014,38 @@ __device__ __forceinline__ double correct_one(const Soa &s, uint32_t i, const do
     s.t[i] = itime;
     s.dt[i] = dt_new;
     return dt_new;
+}
+
+__device__ __forceinline__ double correct_one(const Soa &s, uint32_t i, const double f[6],
+                                              double itime, float eta, double dt_min,
+                                              double dt_max) {
+    double o[6], pr[6];
+    for (int c = 0; c < 6; ++c) o[c] = s.old[c][i];
+    load_predictor(s, i, pr);
+    return correct_store(s, i, o, pr, s.dt[i], f, itime, eta, dt_min, dt_max);
+}
+
+// The corrector's inputs of the particle at place k of the step's active
+// list from the selection's records (Ens::arec): old forces o, predictor
+// pr, step dt1.
+__device__ __forceinline__ void load_arec(const Ens &e, size_t k, double o[6], double pr[6],
+                                          double &dt1) {
+    const size_t np = (size_t)e.nblk * JT;
+    const double4 p = e.arec[k], q = e.arec[np + k], a = e.arec[2 * np + k],
+                  j = e.arec[3 * np + k];
+    pr[0] = p.x;
+    pr[1] = p.y;
+    pr[2] = p.z;
+    pr[3] = q.x;
+    pr[4] = q.y;
+    pr[5] = q.z;
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = a.z;
+    o[3] = j.x;
+    o[4] = j.y;
+    o[5] = j.z;
+    dt1 = a.w;
 }
 
 // Standalone corrector of cluster 0's active list at itime (the ABI call
@@ -1005,9 +1109,11 @@ __device__ __forceinline__ void chunk_sums(const Soa &s, const ClPlan &pl, const
 
 // Finish pass of an update: adds the j-chunk partials of each active
 // particle (chunk_sums) and either stores the forces (FIN_WRITE_F) or
-// applies the corrector at the cluster's block time ATIME (FIN_CORRECT).
-// A particle's chunk sums are spread over L = pl.lanes lanes, so a
-// workgroup finishes BS/L particles with no barrier.
+// applies the corrector at the cluster's block time ATIME (FIN_CORRECT,
+// after a selection that wrote the active particles' records, whose loads
+// go out with those of the partials).  A particle's chunk sums are spread
+// over L = pl.lanes lanes, so a workgroup finishes BS/L particles with no
+// barrier.
 __global__ __launch_bounds__(BS) void k_finish(Soa s, Ens e, const uint32_t *move,
                                               const double *partial, uint32_t mode, float eta,
                                               double dt_min, double dt_max) {
@@ -1018,13 +1124,16 @@ __global__ __launch_bounds__(BS) void k_finish(Soa s, Ens e, const uint32_t *mov
         const uint32_t L = pl.lanes;
         const uint32_t lu = (uint32_t)(u - e.fin_pre[c]);
         const uint32_t nact = e.cs[c].nact;
+        const double T = e.cs[c].atime;
         const uint32_t k = lu * (BS / L) + threadIdx.x / L;
         const bool valid = k < nact;
         const uint32_t i = valid ? move[e.off[c] + k] : 0u;
+        double o[6], pr[6], dt1 = 0.0;
+        if (mode == FIN_CORRECT && valid) load_arec(e, (size_t)e.off[c] + k, o, pr, dt1);
         double f[6];
         chunk_sums(s, pl, partial + e.part_pre[c], k, i, valid, f);
         if (valid && threadIdx.x % L == 0) {
-            if (mode == FIN_CORRECT) correct_one(s, i, f, e.cs[c].atime, eta, dt_min, dt_max);
+            if (mode == FIN_CORRECT) correct_store(s, i, o, pr, dt1, f, T, eta, dt_min, dt_max);
             if (pl.S > 1)
                 for (int q = 0; q < 6; ++q) s.f[q][i] = f[q];
         }
@@ -1070,10 +1179,13 @@ __global__ __launch_bounds__(BS) void k_blockmin_dirty(Soa s, Ens e, uint32_t pa
         cs->tail_pend = 1;
         cs->nact = 0;
     }
-    const uint32_t nd = e.ctl->ndirty[par];
+    // this workgroup's first list entry is read with the count (the list
+    // has nblk entries and the grid is at most nblk)
     const uint32_t *dl = e.dirty + (size_t)par * 3 * e.nblk;
+    uint32_t b = dl[3 * blockIdx.x];
+    const uint32_t nd = e.ctl->ndirty[par];
     for (uint32_t w = blockIdx.x; w < nd; w += gridDim.x) {
-        uint32_t b = dl[3 * w];
+        if (w != blockIdx.x) b = dl[3 * w];
         __syncthreads();
         double m = block_tmin(s, b, red);
         if (threadIdx.x == 0) e.bmin[b] = m;
@@ -1100,12 +1212,20 @@ __global__ __launch_bounds__(BS) void k_finish_blocks(Soa s, Ens e, const uint32
     __shared__ double red[BS / 64];
     __shared__ double tnew[JT];  // t + dt of the block's corrected particles (NaN: not active)
     ClState *cs = &e.cs[0];
-    if (cs->done) return;  // the loop stopped at this step's selection
-    const uint32_t nd = e.ctl->ndirty[par];
     const uint32_t *dl = e.dirty + (size_t)par * 3 * e.nblk;
+    // What the first block needs is requested at once: the loop's stop
+    // flag, the list's count, this workgroup's list entry (read before the
+    // count is known: the list has nblk entries and the grid is at most
+    // nblk; an entry beyond the count is not used), the plan and the block
+    // time.
+    const uint32_t done = cs->done;
+    const uint32_t nd = e.ctl->ndirty[par];
+    uint32_t w = blockIdx.x;
+    uint32_t b = dl[3 * w], k0 = dl[3 * w + 1], cnt = dl[3 * w + 2];
     const ClPlan pl = e.plan[0];
-    const uint32_t L = pl.lanes, per = BS / L;
     const double T = cs->atime;
+    if (done) return;  // the loop stopped at this step's selection
+    const uint32_t L = pl.lanes, per = BS / L;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // this step's loop tail, for the next step's selection (k_select
         // PS_TAIL), k_small_run or k_tail to make; the active count of the
@@ -1116,8 +1236,12 @@ __global__ __launch_bounds__(BS) void k_finish_blocks(Soa s, Ens e, const uint32
         cs->tail_pend = 1;
         cs->nact = 0;
     }
-    for (uint32_t w = blockIdx.x; w < nd; w += gridDim.x) {
-        const uint32_t b = dl[3 * w], k0 = dl[3 * w + 1], cnt = dl[3 * w + 2];
+    for (; w < nd; w += gridDim.x) {
+        if (w != blockIdx.x) {
+            b = dl[3 * w];
+            k0 = dl[3 * w + 1];
+            cnt = dl[3 * w + 2];
+        }
         // t + dt of the block as it stands, loaded ahead of the corrector
         const uint32_t j = b * JT + threadIdx.x;
         const double tdt = s.t[j] + s.dt[j];
@@ -1129,10 +1253,14 @@ __global__ __launch_bounds__(BS) void k_finish_blocks(Soa s, Ens e, const uint32
             const bool valid = kk < cnt;
             const uint32_t k = k0 + (valid ? kk : 0u);
             const uint32_t i = move[k];
+            // the corrector's inputs from the selection's records: their
+            // loads go out with those of the partials
+            double o[6], pr[6], dt1 = 0.0;
+            if (valid) load_arec(e, k, o, pr, dt1);
             double f[6];
             chunk_sums(s, pl, partial, k, i, valid, f);
             if (valid && threadIdx.x % L == 0) {
-                const double dtn = correct_one(s, i, f, T, eta, dt_min, dt_max);
+                const double dtn = correct_store(s, i, o, pr, dt1, f, T, eta, dt_min, dt_max);
                 if (pl.S > 1)
                     for (int q = 0; q < 6; ++q) s.f[q][i] = f[q];
                 tnew[i - b * JT] = T + dtn;
@@ -1810,6 +1938,7 @@ struct h4h_ctx {
     RunCtl *d_ctl = nullptr;
     uint32_t *d_dirty = nullptr;
     double *d_bmin = nullptr;
+    double4 *d_arec = nullptr;   // the step's active particles' records (Ens::arec)
     uint32_t *d_tune = nullptr;  // [64] tuning table
     double *hist_t = nullptr;
     uint32_t *hist_n = nullptr;
@@ -1903,15 +2032,16 @@ int configure_launch(h4h_ctx *ctx) {
 
 uint32_t finish_grid(h4h_ctx *ctx) { return std::min<uint32_t>(1024, std::max<uint32_t>(64, ctx->nblk)); }
 
-// Pair kernel of the current precision over the plan's items.
-int launch_update(h4h_ctx *ctx) {
+// Pair kernel of the current precision over the plan's items.  compact:
+// the step's selection wrote the active particles' records.
+int launch_update(h4h_ctx *ctx, bool compact = false) {
     if (ctx->precision == H4H_FP64_EXACT) {
         hipLaunchKernelGGL(k_update_exact, dim3(ctx->e.grid), dim3(BS), 0, ctx->stream, ctx->s,
                            ctx->e, ctx->move, ctx->e2);
     } else {
         auto *kern = ctx->precision == H4H_MIXED ? k_update_mixed : k_update_fp64;
         hipLaunchKernelGGL(kern, dim3(ctx->e.grid), dim3(BS), 0, ctx->stream, ctx->s, ctx->e,
-                           ctx->move, ctx->partial, ctx->e2);
+                           ctx->move, ctx->partial, ctx->e2, compact ? 1u : 0u);
     }
     H4H_LAUNCHED();
     return H4H_OK;
@@ -2006,13 +2136,15 @@ int ensure_blockmin(h4h_ctx *ctx) {
 // Optional HIP events bracket the update kernel.
 int enqueue_step(h4h_ctx *ctx, bool events, uint32_t par) {
     int rc;
-    if ((rc = launch_select(ctx, PS_PREDICT | PS_SELECT | PS_SAVE_OLD | PS_DEVTIME | PS_DIRTY | PS_TAIL,
+    if ((rc = launch_select(ctx,
+                            PS_PREDICT | PS_SELECT | PS_SAVE_OLD | PS_DEVTIME | PS_DIRTY | PS_TAIL |
+                                PS_COMPACT,
                             0.0, par)))
         return rc;
     // (a single-system context's update kernel makes its own plan)
     if (ctx->K > 1 && (rc = launch_plan(ctx))) return rc;
     if (events) H4H_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-    if ((rc = launch_update(ctx))) return rc;
+    if ((rc = launch_update(ctx, true))) return rc;
     if (events) H4H_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
     if (ctx->K == 1) {
         hipLaunchKernelGGL(k_finish_blocks, dim3(std::min<uint32_t>(ctx->nblk, 1024)), dim3(BS), 0,
@@ -2254,6 +2386,7 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
               hipMalloc(&ctx->d_ctl, sizeof(RunCtl)) == hipSuccess &&
               hipMalloc(&ctx->d_dirty, 6 * (size_t)ctx->nblk * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&ctx->d_bmin, ctx->nblk * sizeof(double)) == hipSuccess &&
+              hipMalloc(&ctx->d_arec, 4 * np * sizeof(double4)) == hipSuccess &&
               hipMalloc(&ctx->d_tune, 64 * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&ctx->hist_t, HIST_CAP * sizeof(double)) == hipSuccess &&
               hipMalloc(&ctx->hist_n, HIST_CAP * sizeof(uint32_t)) == hipSuccess;
@@ -2282,6 +2415,7 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
     e.ctl = ctx->d_ctl;
     e.dirty = ctx->d_dirty;
     e.bmin = ctx->d_bmin;
+    e.arec = ctx->d_arec;
     e.hist_t = ctx->hist_t;
     e.hist_n = ctx->hist_n;
     e.hist_cap = HIST_CAP;
@@ -2488,7 +2622,8 @@ int h4h_destroy(h4h_ctx *ctx) {
         if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
     void *dev[] = {ctx->state, ctx->staging, ctx->partial, ctx->eout, ctx->move, ctx->d_off,
                    ctx->d_cnt, ctx->d_hoff, ctx->d_bcl, ctx->d_cs, ctx->d_plan, ctx->d_pre,
-                   ctx->d_ctl, ctx->d_dirty, ctx->d_bmin, ctx->d_tune, ctx->hist_t, ctx->hist_n};
+                   ctx->d_ctl, ctx->d_dirty, ctx->d_bmin, ctx->d_arec, ctx->d_tune, ctx->hist_t,
+                   ctx->hist_n};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     void *host[] = {ctx->ctl_host, ctx->cs_host, ctx->hist_t_host, ctx->hist_n_host};
@@ -3252,7 +3387,7 @@ int h4h_update_acc_jrk_static(const double *d_p6, const float *d_pm, uint32_t n,
         hipLaunchKernelGGL(k_update_exact, dim3(grid), dim3(BS), 0, st, s, e, move, e2);
     } else {
         auto *kern = precision == H4H_MIXED ? k_update_mixed : k_update_fp64;
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(BS), 0, st, s, e, move, partial, e2);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(BS), 0, st, s, e, move, partial, e2, 0u);
     }
     H4H_LAUNCHED();
     hipLaunchKernelGGL(k_finish, dim3(std::min<uint32_t>(1024, std::max<uint32_t>(64, e.nblk))),
