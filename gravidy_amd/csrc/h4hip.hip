@@ -370,25 +370,6 @@ __device__ __forceinline__ void predict_vals(const double r[3], const double v[3
     b = make_double4(pvel[0], pvel[1], pvel[2], mass_bits(mf));
 }
 
-// Predictor of particle i (time ti) at itime, stored in s.pp / s.pv;
-// returns it too.
-__device__ __forceinline__ void predict_one(const Soa &s, uint32_t i, double ti, double itime,
-                                            double4 *pp = nullptr, double4 *pv = nullptr) {
-#pragma clang fp contract(off)
-    double f[6], r[3], v[3];
-    for (int k = 0; k < 6; ++k) f[k] = s.f[k][i];
-    for (int k = 0; k < 3; ++k) {
-        r[k] = s.r[k][i];
-        v[k] = s.v[k][i];
-    }
-    double4 a, b;
-    predict_vals(r, v, f, itime - ti, (float)s.m[i], a, b);
-    s.pp[i] = a;
-    s.pv[i] = b;
-    if (pp) *pp = a;
-    if (pv) *pv = b;
-}
-
 // PS_TAIL (device loop): each cluster's last block step left its loop tail
 // in its ClState (tail_*); every workgroup of the cluster takes the next
 // block time as the minimum of the cluster's block minima and makes the
@@ -2478,7 +2459,8 @@ int create_impl(h4h_ctx **out, uint32_t K, const uint32_t *sizes, double e2, flo
 // Partial sums: for n <= STATIC_MAX_N with the default launch (grid <= 2048
 // workgroups, queue target 16 items per workgroup, at most 256 i slots per
 // item), a plan with S > 1 chunks has S <= max(s_fill, s_tiles) where
-// s_fill = ceil(grid / n_it) and s_tiles = floor(n_jt / CHUNK_TILES) <= 16:
+// s_fill = ceil(grid / n_it) and s_tiles = floor(n_jt / CHUNK_TILES) <= 16
+// (n_jt <= 256, so the large-set chunk rule does not apply):
 // S n_it iw <= (grid + n_it - 1) iw <= grid BS + n + BS - 1 in the first
 // case and S rows <= 16 (n + BS - 1) in the second, so 6 S rows <=
 // ST_PART doubles.  The call also checks its own plan against ST_PART.
