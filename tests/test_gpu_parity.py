@@ -231,6 +231,32 @@ def test_update_fp64_jsplit_within_tolerance(oracle, n, nact):
     assert np.all(f[rest] == 0.0)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n,nact", [(262144, 8192), (300000, 20000)])
+def test_update_large_active_sets_within_tolerance(oracle, n, nact, precision):
+    """Active sets of 8192 or more over more than 256 j tiles: the plan's
+    chunks of 4 tiles and the finish pass's 4 lanes per particle.  A seeded
+    sample of 512 of the active particles against the oracle (each
+    particle's sum is independent of the others'), and every particle
+    outside the active set keeps zero forces."""
+    rng = np.random.default_rng(n + nact)
+    r, v = ic.plummer(n, 700 + n)
+    move = np.sort(rng.choice(n, size=nact, replace=False)).astype(np.uint32)
+    with Hermite4HIP(r, v, E2, precision=precision) as h:
+        h.set_active(move)
+        h.update_acc_jrk(nact)
+        f = h.download()["f"]
+    sample = np.sort(rng.choice(move, size=512, replace=False)).astype(np.uint32)
+    fo = oracle.update_acc_jrk(sample, p6(r, v), pm_of(r), E2, np.zeros((n, 6)))
+    label = f"update n={n} nact={nact} {precision} (512 sampled)"
+    if precision == "fp64":
+        check_forces(f, fo, sample, p6(r, v), pm_of(r), E2, label)
+    else:
+        check_mixed(f, fo, sample, label)
+    rest = np.setdiff1d(np.arange(n), move)
+    assert np.all(f[rest] == 0.0)
+
+
 def test_autotune_table_is_valid_and_keeps_results_within_tolerance(oracle):
     """h4h_autotune picks a (workgroup share, lane-group width) per log2(nact)
     bucket on the context's own particles; forces computed under the table
