@@ -284,6 +284,32 @@ def same_steps_leg(name, r, v, e2, dt_min, cpu_budget_s, t_window, interval):
     return out
 
 
+def plummer_run(t_end, n=262144):
+    """The large Plummer configuration as a whole run: n equal-mass
+    Plummer stars (seed n), eps = 1e-4, dt in [2^-23, 2^-3], to t_end, FP64
+    and mixed from identical initial conditions (device loop, energy every
+    0.125)."""
+    from gravidy_amd import Hermite4HIP, ic, softening_e2
+    r, v = ic.plummer(n, n)
+    e2 = softening_e2(1e-4)
+    out = {"workload": f"plummer_n{n}_eps1e-4_eta0.01", "t_end": t_end}
+    for precision in ("mixed", "fp64"):
+        t0 = time.perf_counter()
+        with Hermite4HIP(r, v, e2, precision=precision) as h:
+            run = h.integration(t_end, interval=0.125,
+                                on_output=lambda _h, t: log(f"[bench] N={n} {precision} t={t:.6f} "
+                                                            f"{time.perf_counter() - t0:.1f} s"))
+        el = time.perf_counter() - t0
+        e = run["energy"]
+        out[precision] = {"seconds": el, "block_steps": int(len(run["step_time"])),
+                          "interactions": int(run["interactions"]),
+                          "pairs_per_s": run["interactions"] / el,
+                          "mean_nact": float(run["step_nact"].mean()),
+                          "cum_energy_error": float(abs((e[-1] - e[0]) / e[0]))}
+    out["mixed_speedup"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    return out
+
+
 def salpeter_bh_run(t_end):
     """The large mixed-precision configuration as a whole run: 1048576
     Salpeter-mass Plummer stars plus a central particle of mass 0.01,
@@ -468,6 +494,9 @@ def main():
     ap.add_argument("--salpeter-bh-run", type=float, default=0.0, metavar="T_END",
                     help="only run the N=1048577 Salpeter + central-mass configuration to T_END "
                          "in mixed and FP64 (whole runs, minutes)")
+    ap.add_argument("--plummer-run", type=float, default=0.0, metavar="T_END",
+                    help="only run the N (default 262144) Plummer configuration to T_END in "
+                         "mixed and FP64 (whole runs, energy every 0.125)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -485,6 +514,9 @@ def main():
         return
     if args.salpeter_bh_run > 0:
         print(json.dumps({"salpeter_bh_run": salpeter_bh_run(args.salpeter_bh_run)}), flush=True)
+        return
+    if args.plummer_run > 0:
+        print(json.dumps({"plummer_run": plummer_run(args.plummer_run, args.n)}), flush=True)
         return
 
     from gravidy_amd import Hermite4HIP, ic, softening_e2
