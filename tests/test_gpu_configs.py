@@ -216,7 +216,7 @@ def test_salpeter_bh_full_size_initial_forces_conserve_momentum(salpeter_bh_full
         assert net <= 1e-12 * scale, (name, net, scale)
 
 
-@pytest.mark.parametrize("precision", ["exact", "fp64"])
+@pytest.mark.parametrize("precision", ["exact", "fp64", "mixed"])
 def test_python_restart_is_bit_exact(tmp_path, precision):
     """Hermite4HIP.integration(checkpoint=..., restart=...): t = 0 -> 0.25,
     then a new context continues from the checkpoint to 0.5; schedule and

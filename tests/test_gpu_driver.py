@@ -94,7 +94,7 @@ def test_driver_rejects_bad_input(tmp_path):
     assert proc.returncode != 0 and "No such file" in proc.stderr
 
 
-@pytest.mark.parametrize("precision", ["exact", "fp64"])
+@pytest.mark.parametrize("precision", ["exact", "fp64", "mixed"])
 def test_driver_restart_from_checkpoint_is_bit_exact(tmp_path, precision):
     """A run to t=0.5 in one go and a run to t=0.25 continued from its
     binary checkpoint to t=0.5 end with byte-identical checkpoints (state,
