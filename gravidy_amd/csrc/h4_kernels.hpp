@@ -82,8 +82,24 @@ struct ClPlan {
     uint32_t rows;      // n_it * iw
     uint32_t lanes;     // finish lanes per particle (finish_lanes(S, nact))
     uint32_t grid;      // workgroup share given to decompose()
-    uint32_t pad;
+    uint32_t cmaj;      // 1: partials chunk-major (partial_at)
 };
+
+// Place of the j-chunk partial sum of component q, chunk sc, active
+// particle k in a plan's partial buffer (6 S rows doubles).  Component and
+// chunk major, particle minor (a work item writes its particles' sums of
+// one chunk as contiguous runs), unless the finish pass gives each
+// particle 16 or more lanes and the plan has at least 64 i slots: then
+// chunk minor, so that those lanes read whole 128-byte lines of one
+// particle's chunks rather than one double of each of 16 or more lines
+// (the items' scattered writes then cost less than the finish saves; with
+// fewer i slots they cost more).  Either way the finish adds the chunks in
+// the same order, so the sums do not depend on the layout.
+__host__ __device__ inline size_t partial_at(const ClPlan &pl, uint32_t q, uint32_t sc,
+                                             uint32_t k) {
+    return pl.cmaj ? ((size_t)q * pl.rows + k) * pl.S + sc
+                   : ((size_t)q * pl.S + sc) * pl.rows + k;
+}
 
 // Loop control shared by all clusters.
 struct RunCtl {

@@ -189,7 +189,7 @@ __global__ __launch_bounds__(BS) void k_identity_active(uint32_t *move, Ens e) {
 __device__ __forceinline__ ClPlan plan_cluster(const Ens &e, uint32_t c, uint32_t nact, double W,
                                                uint32_t exact) {
     ClPlan pl;
-    pl.pad = 0;
+    pl.cmaj = 0;
     if (exact) {
         pl.iw = BS;
         pl.S = 1;
@@ -214,6 +214,7 @@ __device__ __forceinline__ ClPlan plan_cluster(const Ens &e, uint32_t c, uint32_
         pl.n_it = d.n_it;
         pl.rows = d.rows;
         pl.lanes = finish_lanes(d.S, nact);
+        pl.cmaj = (d.S > 1 && pl.lanes >= 16 && d.rows >= 64) ? 1u : 0u;
     }
     return pl;
 }
@@ -712,8 +713,8 @@ __global__ __launch_bounds__(BS) void k_update_exact(Soa s, Ens e, const uint32_
 
 // One work item of the FP64 / mixed update: i tile `it` of a cluster
 // against j chunk `sc`.  With one chunk the result goes to f[i]; with
-// several, the chunk sums go to partial[(comp*S + sc)*rows + k] and the
-// finish pass adds them in chunk order.
+// several, the chunk sums go to partial[partial_at(pl, comp, sc, k)] and
+// the finish pass adds them in chunk order.
 template <int IW, bool MIXED>
 __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint32_t jt_begin,
                                             uint32_t jt_end, uint32_t it, uint32_t sc,
@@ -809,8 +810,11 @@ __device__ __forceinline__ void update_item(const Soa &s, const ClPlan &pl, uint
         if (pl.S == 1) {
             for (int q = 0; q < 6; ++q) s.f[q][i] = fo[q] * scale;
         } else {
-            for (int q = 0; q < 6; ++q)
-                partial[((size_t)q * pl.S + sc) * pl.rows + k] = fo[q] * scale;
+            // partial_at(pl, q, sc, k), with the layout's strides
+            const size_t kst = pl.cmaj ? pl.S : 1u, cst = pl.cmaj ? 1u : pl.rows;
+            double *pq = partial + (size_t)sc * cst + (size_t)k * kst;
+            const size_t qs = (size_t)pl.S * pl.rows;
+            for (int q = 0; q < 6; ++q) pq[q * qs] = fo[q] * scale;
         }
     }
 }
@@ -1062,8 +1066,10 @@ __device__ __forceinline__ void chunk_sums(const Soa &s, const ClPlan &pl, const
     // together (a few memory latencies per round instead of one per chunk;
     // S <= 1024 takes two rounds at L = 64).
     constexpr uint32_t U = 8;
-    const double *pk = part + k;
+    // stride between consecutive chunks and between components (partial_at)
+    const size_t cs = pl.cmaj ? 1 : pl.rows;
     const size_t qs = (size_t)pl.S * pl.rows;
+    const double *pk = part + (pl.cmaj ? (size_t)k * pl.S : (size_t)k);
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (valid) {
         for (uint32_t s0 = sl; s0 < pl.S; s0 += U * L) {
@@ -1073,7 +1079,7 @@ __device__ __forceinline__ void chunk_sums(const Soa &s, const ClPlan &pl, const
 #pragma unroll
                 for (uint32_t u = 0; u < U; ++u) {
                     const uint32_t sc = s0 + u * L;
-                    v[q][u] = sc < pl.S ? pk[q * qs + (size_t)sc * pl.rows] : 0.0;
+                    v[q][u] = sc < pl.S ? pk[q * qs + (size_t)sc * cs] : 0.0;
                 }
 #pragma unroll
             for (int q = 0; q < 6; ++q)
