@@ -3,7 +3,7 @@
 // active count of the tested system sizes and every lane-group width, the
 // chunks of a plan cover the j tiles exactly and the plan's j-split
 // partials fit the bound the library allocates; random ensembles check the
-// summed bound.  Built and run by tests/test_plan_bounds.py; exit status 0
+// summed bound; partial_at is one to one in both layouts.  Built and run by tests/test_plan_bounds.py; exit status 0
 // means every case held.
 #include "h4_kernels.hpp"
 
@@ -73,7 +73,32 @@ static void check_ensembles(uint32_t trials, uint32_t grid) {
     }
 }
 
+// partial_at maps (component, chunk, particle slot) one to one onto the
+// plan's 6 S rows doubles in both layouts.
+static void check_layouts() {
+    const uint32_t shapes[][2] = {{2, 4}, {3, 64}, {32, 320}, {342, 320}, {1024, 8}, {17, 256}};
+    for (const auto &sh : shapes)
+        for (uint32_t cm = 0; cm < 2; ++cm) {
+            ClPlan pl{};
+            pl.S = sh[0];
+            pl.rows = sh[1];
+            pl.cmaj = cm;
+            const size_t total = 6ull * pl.S * pl.rows;
+            std::vector<unsigned char> seen(total, 0);
+            bool ok = true;
+            for (uint32_t q = 0; q < 6; ++q)
+                for (uint32_t sc = 0; sc < pl.S; ++sc)
+                    for (uint32_t k = 0; k < pl.rows; ++k) {
+                        const size_t at = partial_at(pl, q, sc, k);
+                        if (at >= total || seen[at]++) ok = false;
+                    }
+            if (!ok && failures++ < 10)
+                std::printf("FAIL partial_at S=%u rows=%u cmaj=%u\n", pl.S, pl.rows, cm);
+        }
+}
+
 int main() {
+    check_layouts();
     const uint32_t sizes[] = {1, 2, 3, 255, 256, 257, 1000, 1024, 4097, 65536};
     const uint32_t grids[] = {1, 7, 256, 2048, 4096};
     const uint32_t splits[] = {1, 16, 1024, 1u << 30};
