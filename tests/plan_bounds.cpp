@@ -3,8 +3,8 @@
 // active count of the tested system sizes and every lane-group width, the
 // chunks of a plan cover the j tiles exactly and the plan's j-split
 // partials fit the bound the library allocates; random ensembles check the
-// summed bound; partial_at is one to one in both layouts.  Built and run by tests/test_plan_bounds.py; exit status 0
-// means every case held.
+// summed bound; partial_at is one to one in both layouts.  Built and run
+// by tests/test_plan_bounds.py; exit status 0 means every case held.
 #include "h4_kernels.hpp"
 
 #include <cstdio>
