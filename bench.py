@@ -606,10 +606,10 @@ def main():
     configs = None
     if not args.no_configs:
         configs = {}
-        log("[bench] configuration legs: Plummer N=262144 (to t=1/16), "
+        log("[bench] configuration legs: Plummer N=262144 (the whole run to t=1/4), "
             "Salpeter + central mass N=1048577 (to t=2^-8)")
         configs["plummer_n262144"] = same_steps_leg(
-            "plummer_n262144_eps1e-4", r, v, e2, 2.0 ** -23, args.cpu_budget, 0.0625, 0.03125)
+            "plummer_n262144_eps1e-4", r, v, e2, 2.0 ** -23, args.cpu_budget, 0.25, 0.125)
         from gravidy_amd import ic as _ic
         rb, vb = _ic.plummer_salpeter_bh(1048576, 1048576)
         configs["salpeter_bh_n1048577"] = same_steps_leg(
